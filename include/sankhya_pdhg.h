@@ -1,0 +1,80 @@
+/*
+ * sankhya_pdhg.h - C ABI of the MI355X PDHG engine for foreign callers (Python ctypes, tests,
+ * the benchmark).
+ *
+ * It plays the role of the reference's C API for this one path: sankhya_solve
+ * (include/sankhya/sankhya.h:248, src/api/c_api.cpp:563) over a model built from arrays, with
+ * options given as "name=value" text the way sankhya_options_set_* take them
+ * (include/sankhya/sankhya.h:228-233). Plain C types only; caller-owned buffers.
+ */
+#ifndef SANKHYA_PDHG_H
+#define SANKHYA_PDHG_H
+
+#include <stdint.h>
+
+#include "pdhg_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sankhya_model sankhya_model;
+
+/* A model from CSC arrays (rows ascending within each column); costs in the model's own
+ * sense; maximize = 0 or 1. Returns NULL on a malformed pattern. */
+sankhya_model* sankhya_model_new(int32_t m, int32_t n, int64_t nnz, const int32_t* colptr,
+                                 const int32_t* rowidx, const double* val, const double* cost,
+                                 const double* col_lower, const double* col_upper,
+                                 const double* row_lower, const double* row_upper,
+                                 double objective_offset, int32_t maximize);
+
+/* A deterministic synthetic model: kind 0 uniform-equality, 1 zipf, 2 blend (the
+ * constructions are documented with the generator). */
+sankhya_model* sankhya_model_generate(int32_t kind, int32_t m, int32_t n, int32_t per_col,
+                                      double param, uint64_t seed);
+
+void sankhya_model_dims(const sankhya_model* model, int32_t* m, int32_t* n, int64_t* nnz);
+
+/* Copies the model out (any pointer may be NULL). */
+void sankhya_model_export(const sankhya_model* model, int32_t* colptr, int32_t* rowidx,
+                          double* val, double* cost, double* col_lower, double* col_upper,
+                          double* row_lower, double* row_upper, double* objective_offset,
+                          int32_t* maximize);
+
+void sankhya_model_free(sankhya_model* model);
+
+typedef struct {
+  int32_t status; /* sankhya::SolveStatus */
+  int64_t iterations, attempts, restarts, ticks, device_bytes;
+  double objective, dual_bound, solve_seconds, setup_seconds, loop_seconds, spectral_norm;
+  double primal_infeasibility, dual_infeasibility, complementarity_violation, relative_gap;
+  double final_primal, final_dual, final_gap;
+  char algorithm[32];
+  char message[1024];
+} sankhya_result;
+
+/* Solve with the MI355X engine. options: "name=value" pairs separated by ';' or ','
+ * (NULL or "" for defaults; log_to_console defaults to false here). Any output pointer may
+ * be NULL. Returns 0, or -1 when an option is rejected (message says which). */
+int sankhya_pdhg_solve(const sankhya_model* model, const char* options, double* col_value,
+                       double* row_dual, double* col_dual, double* row_activity,
+                       sankhya_result* result);
+
+/* The host preconditioner on its own (src/la/scaling.cpp:111-159): Dr (m), Dc (n), the scaled
+ * CSC values (nnz), scaled cost/bounds. min_cost is sense * c. */
+int sankhya_pdhg_scaling(const sankhya_model* model, int ruiz_iterations, double* row,
+                         double* column, double* scaled_val, double* cost_hat,
+                         double* col_lower_hat, double* col_upper_hat, double* row_lower_hat,
+                         double* row_upper_hat);
+
+/* Opens a device handle holding this model exactly as the engine uploads it (scaled, both
+ * CSR layouts), for the pdhg_dev_* test hooks. Also returns the unscaled min-space cost norm
+ * and bound norm through the handle. */
+int sankhya_pdhg_device_open(const sankhya_model* model, int device, int precision,
+                             pdhg_dev** out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,31 @@
+// SPDX-License-Identifier: Apache-2.0
+// Restarted PDHG, the first-order LP engine (reference: include/sankhya/pdhg.hpp:9-18).
+//
+// In this build the engine runs on one MI355X (gfx950) GPU. There is no CPU engine behind
+// this entry point: without a usable device the solve returns kNumericalError with a message
+// that says so.
+#pragma once
+
+#include "sankhya/logging.hpp"
+#include "sankhya/model.hpp"
+#include "sankhya/options.hpp"
+
+namespace sankhya::pdhg {
+
+/// Solve an LP with restarted primal-dual hybrid gradient. Never throws.
+[[nodiscard]] Solution solve_pdhg(const Model& model, const Options& options, Logger& logger,
+                                  SolveControl* control = nullptr);
+
+}  // namespace sankhya::pdhg
+
+namespace sankhya::gpu {
+
+/// The same engine under the name the reference's dispatcher uses for its GPU backend
+/// (src/gpu/pdhg_gpu.hpp:19-20).
+[[nodiscard]] Solution solve_pdhg_gpu(const Model& model, const Options& options,
+                                      Logger& logger, SolveControl* control = nullptr);
+
+/// Is a gfx950 device present? Fills `description` (reference src/gpu/device.hpp).
+[[nodiscard]] bool device_available(std::string* description);
+
+}  // namespace sankhya::gpu

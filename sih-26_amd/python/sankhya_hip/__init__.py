@@ -1,0 +1,341 @@
+"""Python bindings of the MI355X restarted-PDHG engine (ctypes over include/sankhya_pdhg.h and
+include/pdhg_hip.h).
+
+Mirrors the shape of the reference's ctypes bindings (bindings/python/sankhya/_library.py and
+__init__.py:485-699): a Model built from arrays, `solve()` with options as keyword arguments,
+and a Solution with the reference's field names. The shared library is built in-tree
+(sih-26_amd/lib/libsankhya_hip.so); importing this module without it raises, and nothing here
+falls back to a CPU solver.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Dict, Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "lib", "libsankhya_hip.so"))
+
+STATUS_NAMES = [
+    "not_solved", "optimal", "infeasible", "unbounded", "infeasible_or_unbounded", "feasible",
+    "iteration_limit", "time_limit", "node_limit", "numerical_error", "model_error",
+    "interrupted",
+]
+PRECISION = {"fp64": 0, "mixed": 1}
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32),
+        ("iterations", C.c_int64), ("attempts", C.c_int64), ("restarts", C.c_int64),
+        ("ticks", C.c_int64), ("device_bytes", C.c_int64),
+        ("objective", C.c_double), ("dual_bound", C.c_double), ("solve_seconds", C.c_double),
+        ("setup_seconds", C.c_double), ("loop_seconds", C.c_double),
+        ("spectral_norm", C.c_double),
+        ("primal_infeasibility", C.c_double), ("dual_infeasibility", C.c_double),
+        ("complementarity_violation", C.c_double), ("relative_gap", C.c_double),
+        ("final_primal", C.c_double), ("final_dual", C.c_double), ("final_gap", C.c_double),
+        ("algorithm", C.c_char * 32), ("message", C.c_char * 1024),
+    ]
+
+
+class DevState(C.Structure):
+    _fields_ = [
+        ("iteration", C.c_int64), ("attempts", C.c_int64), ("restarts", C.c_int64),
+        ("ticks", C.c_int64), ("done", C.c_int32), ("limit_hit", C.c_int32),
+        ("halted", C.c_int32), ("eta", C.c_double), ("omega", C.c_double),
+        ("best_worst", C.c_double), ("best_primal", C.c_double), ("best_dual", C.c_double),
+        ("best_dual_objective", C.c_double),
+    ]
+
+
+class DevParams(C.Structure):
+    _fields_ = [
+        ("tolerance", C.c_double), ("iteration_limit", C.c_int64), ("restart", C.c_int32),
+        ("stop_at_request", C.c_int32), ("spectral_norm", C.c_double),
+        ("chunk_attempts", C.c_int32),
+    ]
+
+
+class Tick(C.Structure):
+    _fields_ = [
+        ("iteration", C.c_int64), ("primal_objective", C.c_double), ("primal", C.c_double),
+        ("dual", C.c_double), ("gap", C.c_double), ("restarted", C.c_int32),
+        ("chose_average", C.c_int32),
+    ]
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def load_library() -> C.CDLL:
+    """Load lib/libsankhya_hip.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    lib.sankhya_model_new.restype = vp
+    lib.sankhya_model_new.argtypes = [C.c_int32, C.c_int32, C.c_int64, _i32p, _i32p, _f64p, _f64p,
+                                      _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_int32]
+    lib.sankhya_model_generate.restype = vp
+    lib.sankhya_model_generate.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                           C.c_uint64]
+    lib.sankhya_model_dims.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int64)]
+    lib.sankhya_model_export.argtypes = [vp, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    lib.sankhya_model_free.argtypes = [vp]
+    lib.sankhya_pdhg_solve.restype = C.c_int
+    lib.sankhya_pdhg_solve.argtypes = [vp, C.c_char_p, vp, vp, vp, vp, C.POINTER(Result)]
+    lib.sankhya_pdhg_scaling.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                         _f64p, _f64p]
+    lib.sankhya_pdhg_device_open.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
+    lib.pdhg_dev_probe.argtypes = [C.c_int, C.c_char_p, C.c_int]
+    lib.pdhg_dev_destroy.argtypes = [vp]
+    lib.pdhg_dev_last_error.restype = C.c_char_p
+    lib.pdhg_dev_last_error.argtypes = [vp]
+    lib.pdhg_dev_spmv.argtypes = [vp, C.c_int, _f64p, _f64p]
+    lib.pdhg_dev_spectral_norm.argtypes = [vp, _f64p, C.c_int, C.POINTER(C.c_double)]
+    lib.pdhg_dev_evaluate.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, _f64p]
+    lib.pdhg_dev_start.argtypes = [vp, C.POINTER(DevParams)]
+    lib.pdhg_dev_run.argtypes = [vp, C.c_int, C.POINTER(DevState)]
+    lib.pdhg_dev_ticks.argtypes = [vp, C.c_int64, C.POINTER(Tick), C.c_int64, C.POINTER(C.c_int64)]
+    lib.pdhg_dev_finish.argtypes = [vp, _f64p, _f64p, _f64p, _f64p]
+    lib.pdhg_dev_time_kernels.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.pdhg_dev_bytes.restype = C.c_int64
+    lib.pdhg_dev_bytes.argtypes = [vp]
+    lib.pdhg_dev_inject_fault.argtypes = [vp, C.c_int]
+    _lib = lib
+    return lib
+
+
+def device_description(device: int = 0):
+    """(available, text) for a gfx950 device, as the engine's own probe sees it."""
+    lib = load_library()
+    buf = C.create_string_buffer(256)
+    rc = lib.pdhg_dev_probe(device, buf, 256)
+    return rc == 0, buf.value.decode()
+
+
+class Model:
+    """An LP:  min/max offset + c.x  s.t.  row_lower <= A x <= row_upper,  col_lower <= x <= col_upper.
+
+    A is CSC (colptr, rowidx ascending per column, val) like the reference's SparseMatrix.
+    """
+
+    def __init__(self, handle: int, owned: bool = True):
+        self._lib = load_library()
+        if not handle:
+            raise ValueError("model construction failed")
+        self._h = C.c_void_p(handle)
+        self._owned = owned
+
+    @classmethod
+    def from_csc(cls, m, n, colptr, rowidx, val, cost, col_lower, col_upper, row_lower,
+                 row_upper, objective_offset=0.0, maximize=False) -> "Model":
+        lib = load_library()
+        a = lambda v, t=np.float64: np.ascontiguousarray(v, dtype=t)
+        colptr, rowidx = a(colptr, np.int32), a(rowidx, np.int32)
+        h = lib.sankhya_model_new(int(m), int(n), int(colptr[-1]) if n else 0, colptr, rowidx, a(val),
+                                  a(cost), a(col_lower), a(col_upper), a(row_lower), a(row_upper),
+                                  float(objective_offset), 1 if maximize else 0)
+        return cls(h)
+
+    @classmethod
+    def from_dense(cls, rows, row_lower, row_upper, cost, col_lower=None, col_upper=None,
+                   objective_offset=0.0, maximize=False) -> "Model":
+        A = np.asarray(rows, dtype=np.float64).reshape(len(row_lower), len(cost))
+        m, n = A.shape
+        colptr, rowidx, val = [0], [], []
+        for j in range(n):
+            for i in range(m):
+                if A[i, j] != 0.0:
+                    rowidx.append(i)
+                    val.append(A[i, j])
+            colptr.append(len(rowidx))
+        cl = np.zeros(n) if col_lower is None else col_lower
+        cu = np.full(n, np.inf) if col_upper is None else col_upper
+        return cls.from_csc(m, n, colptr, rowidx, val, cost, cl, cu, row_lower, row_upper,
+                            objective_offset, maximize)
+
+    @classmethod
+    def generate(cls, kind: str, m: int, n: int, per_col: int, seed: int, param: float = 1.5) -> "Model":
+        kinds = {"uniform": 0, "zipf": 1, "blend": 2}
+        lib = load_library()
+        return cls(lib.sankhya_model_generate(kinds[kind], m, n, per_col, param, seed))
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and getattr(self, "_h", None):
+            self._lib.sankhya_model_free(self._h)
+            self._h = None
+
+    @property
+    def dims(self):
+        m, n, nnz = C.c_int32(), C.c_int32(), C.c_int64()
+        self._lib.sankhya_model_dims(self._h, C.byref(m), C.byref(n), C.byref(nnz))
+        return m.value, n.value, nnz.value
+
+    def arrays(self) -> Dict[str, np.ndarray]:
+        m, n, nnz = self.dims
+        d = dict(colptr=np.zeros(n + 1, np.int32), rowidx=np.zeros(max(nnz, 1), np.int32),
+                 val=np.zeros(max(nnz, 1)), cost=np.zeros(max(n, 1)), col_lower=np.zeros(max(n, 1)),
+                 col_upper=np.zeros(max(n, 1)), row_lower=np.zeros(max(m, 1)),
+                 row_upper=np.zeros(max(m, 1)))
+        off, mx = C.c_double(), C.c_int32()
+        self._lib.sankhya_model_export(self._h, d["colptr"], d["rowidx"], d["val"], d["cost"],
+                                       d["col_lower"], d["col_upper"], d["row_lower"],
+                                       d["row_upper"], C.byref(off), C.byref(mx))
+        for k, ln in (("rowidx", nnz), ("val", nnz), ("cost", n), ("col_lower", n),
+                      ("col_upper", n), ("row_lower", m), ("row_upper", m)):
+            d[k] = d[k][:ln].copy()
+        d.update(m=m, n=n, nnz=nnz, objective_offset=off.value, maximize=bool(mx.value))
+        return d
+
+
+@dataclass
+class Solution:
+    status: str
+    objective: float
+    dual_bound: float
+    iterations: int
+    col_value: np.ndarray
+    row_dual: np.ndarray
+    col_dual: np.ndarray
+    row_activity: np.ndarray
+    algorithm: str
+    message: str
+    solve_seconds: float
+    primal_infeasibility: float
+    dual_infeasibility: float
+    complementarity_violation: float
+    relative_gap: float
+    stats: Dict[str, float] = field(default_factory=dict)
+
+
+def _options_text(options: Dict[str, object]) -> bytes:
+    parts = []
+    for k, v in options.items():
+        if isinstance(v, bool):
+            v = "true" if v else "false"
+        parts.append(f"{k}={v}")
+    return ";".join(parts).encode()
+
+
+def solve(model: Model, **options) -> Solution:
+    """Solve with the MI355X PDHG engine. Options are the registry's names (pdhg_tolerance,
+    iteration_limit, pdhg_stop_at_request, pdhg_precision, ...)."""
+    lib = load_library()
+    m, n, _ = model.dims
+    x, y, d, act = np.zeros(max(n, 1)), np.zeros(max(m, 1)), np.zeros(max(n, 1)), np.zeros(max(m, 1))
+    res = Result()
+    rc = lib.sankhya_pdhg_solve(model._h, _options_text(options), x.ctypes.data, y.ctypes.data,
+                                d.ctypes.data, act.ctypes.data, C.byref(res))
+    if rc != 0:
+        raise ValueError(res.message.decode())
+    stats = {k: getattr(res, k) for k in ("attempts", "restarts", "ticks", "device_bytes",
+                                          "setup_seconds", "loop_seconds", "spectral_norm",
+                                          "final_primal", "final_dual", "final_gap")}
+    return Solution(STATUS_NAMES[res.status], res.objective, res.dual_bound, res.iterations,
+                    x[:n], y[:m], d[:n], act[:m], res.algorithm.decode(), res.message.decode(),
+                    res.solve_seconds, res.primal_infeasibility, res.dual_infeasibility,
+                    res.complementarity_violation, res.relative_gap, stats)
+
+
+def host_scaling(model: Model, ruiz: int = 10) -> Dict[str, np.ndarray]:
+    """The engine's host preconditioner (Dr, Dc, scaled CSC values, scaled vectors)."""
+    lib = load_library()
+    m, n, nnz = model.dims
+    out = dict(row=np.zeros(max(m, 1)), column=np.zeros(max(n, 1)), val=np.zeros(max(nnz, 1)),
+               cost=np.zeros(max(n, 1)), col_lower=np.zeros(max(n, 1)), col_upper=np.zeros(max(n, 1)),
+               row_lower=np.zeros(max(m, 1)), row_upper=np.zeros(max(m, 1)))
+    lib.sankhya_pdhg_scaling(model._h, ruiz, out["row"], out["column"], out["val"], out["cost"],
+                             out["col_lower"], out["col_upper"], out["row_lower"], out["row_upper"])
+    for k, ln in (("row", m), ("column", n), ("val", nnz), ("cost", n), ("col_lower", n),
+                  ("col_upper", n), ("row_lower", m), ("row_upper", m)):
+        out[k] = out[k][:ln]
+    return out
+
+
+class Device:
+    """A device handle holding one model as the engine uploads it; exposes the pdhg_dev_* hooks."""
+
+    def __init__(self, model: Model, precision: str = "fp64", device: int = 0):
+        self._lib = load_library()
+        self.model = model
+        self.m, self.n, self.nnz = model.dims
+        h = C.c_void_p()
+        rc = self._lib.sankhya_pdhg_device_open(model._h, device, PRECISION[precision], C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"device open failed with code {rc}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pdhg_dev_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {self._lib.pdhg_dev_last_error(self._h).decode()}")
+
+    def spmv(self, x: np.ndarray, transpose: bool = False) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(max(self.n if transpose else self.m, 1))
+        self._check(self._lib.pdhg_dev_spmv(self._h, 1 if transpose else 0, x, out), "spmv")
+        return out[: (self.n if transpose else self.m)]
+
+    def spectral_norm(self, v0: np.ndarray, iterations: int = 30) -> float:
+        r = C.c_double()
+        self._check(self._lib.pdhg_dev_spectral_norm(self._h, np.ascontiguousarray(v0, np.float64),
+                                                     iterations, C.byref(r)), "spectral_norm")
+        return r.value
+
+    def evaluate(self, x: np.ndarray, y: np.ndarray):
+        act, red, res = np.zeros(max(self.m, 1)), np.zeros(max(self.n, 1)), np.zeros(9)
+        self._check(self._lib.pdhg_dev_evaluate(self._h, np.ascontiguousarray(x, np.float64),
+                                                np.ascontiguousarray(y, np.float64), act, red, res),
+                    "evaluate")
+        return act[: self.m], red[: self.n], res
+
+    def start(self, tolerance=1e-4, iteration_limit=1000000, restart=True, stop_at_request=False,
+              spectral_norm=1.0, chunk=0):
+        p = DevParams(tolerance, iteration_limit, 1 if restart else 0, 1 if stop_at_request else 0,
+                      spectral_norm, chunk)
+        self._check(self._lib.pdhg_dev_start(self._h, C.byref(p)), "start")
+
+    def run(self, chunks: int = 1) -> DevState:
+        st = DevState()
+        self._check(self._lib.pdhg_dev_run(self._h, chunks, C.byref(st)), "run")
+        return st
+
+    def ticks(self, first: int = 0, max_count: int = 1 << 16):
+        buf = (Tick * max_count)()
+        got = C.c_int64()
+        self._check(self._lib.pdhg_dev_ticks(self._h, first, buf, max_count, C.byref(got)), "ticks")
+        return [buf[k] for k in range(got.value)]
+
+    def finish(self):
+        x, y, red, res = np.zeros(max(self.n, 1)), np.zeros(max(self.m, 1)), np.zeros(max(self.n, 1)), np.zeros(9)
+        self._check(self._lib.pdhg_dev_finish(self._h, x, y, red, res), "finish")
+        return x[: self.n], y[: self.m], red[: self.n], res
+
+    def time_kernels(self, reps: int = 20):
+        a, b = C.c_double(), C.c_double()
+        self._check(self._lib.pdhg_dev_time_kernels(self._h, reps, C.byref(a), C.byref(b)), "time_kernels")
+        return a.value, b.value
+
+    @property
+    def bytes(self) -> int:
+        return int(self._lib.pdhg_dev_bytes(self._h))

@@ -1,0 +1,250 @@
+// SPDX-License-Identifier: Apache-2.0
+// C ABI for foreign callers (include/sankhya_pdhg.h). Reference counterpart: the C API over
+// solve(), src/api/c_api.cpp:563, which likewise catches everything and returns codes.
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "../../include/sankhya_pdhg.h"
+#include "engine_stats.hpp"
+#include "generate.hpp"
+#include "sankhya/pdhg.hpp"
+#include "scaling.hpp"
+
+struct sankhya_model {
+  sankhya::Model model;
+};
+
+using namespace sankhya;
+
+namespace {
+
+void copy_text(char* dst, std::size_t cap, const std::string& s) {
+  const std::size_t k = std::min(cap - 1, s.size());
+  std::memcpy(dst, s.data(), k);
+  dst[k] = '\0';
+}
+
+bool apply_options(Options* o, const char* text, std::string* error) {
+  if (!text) return true;
+  std::string s(text);
+  std::size_t pos = 0;
+  while (pos < s.size()) {
+    std::size_t end = s.find_first_of(";,", pos);
+    if (end == std::string::npos) end = s.size();
+    const std::string item = s.substr(pos, end - pos);
+    pos = end + 1;
+    if (item.find_first_not_of(" \t") == std::string::npos) continue;
+    const std::size_t eq = item.find('=');
+    if (eq == std::string::npos) {
+      *error = "option without '=': " + item;
+      return false;
+    }
+    auto trim = [](std::string t) {
+      const auto b = t.find_first_not_of(" \t");
+      const auto e = t.find_last_not_of(" \t");
+      return b == std::string::npos ? std::string() : t.substr(b, e - b + 1);
+    };
+    if (!o->set_from_string(trim(item.substr(0, eq)), trim(item.substr(eq + 1)), error)) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+sankhya_model* sankhya_model_new(int32_t m, int32_t n, int64_t nnz, const int32_t* colptr,
+                                 const int32_t* rowidx, const double* val, const double* cost,
+                                 const double* col_lower, const double* col_upper,
+                                 const double* row_lower, const double* row_upper,
+                                 double objective_offset, int32_t maximize) {
+  if (m < 0 || n < 0 || nnz < 0 || !colptr || colptr[0] != 0 || colptr[n] != nnz) return nullptr;
+  for (int32_t j = 0; j < n; ++j) {
+    if (colptr[j + 1] < colptr[j]) return nullptr;
+    for (int32_t k = colptr[j]; k < colptr[j + 1]; ++k) {
+      if (rowidx[k] < 0 || rowidx[k] >= m) return nullptr;
+      if (k > colptr[j] && rowidx[k] <= rowidx[k - 1]) return nullptr;
+    }
+  }
+  auto* h = new sankhya_model();
+  Model& md = h->model;
+  md.resize_columns(n);
+  md.resize_rows(m);
+  for (int32_t j = 0; j < n; ++j) {
+    md.col_cost[j] = cost ? cost[j] : 0.0;
+    md.col_lower[j] = col_lower ? normalize_infinity(col_lower[j]) : 0.0;
+    md.col_upper[j] = col_upper ? normalize_infinity(col_upper[j]) : kInfinity;
+  }
+  for (int32_t i = 0; i < m; ++i) {
+    md.row_lower[i] = row_lower ? normalize_infinity(row_lower[i]) : -kInfinity;
+    md.row_upper[i] = row_upper ? normalize_infinity(row_upper[i]) : kInfinity;
+  }
+  md.objective_offset = objective_offset;
+  md.sense = maximize ? ObjSense::kMaximize : ObjSense::kMinimize;
+  md.matrix.assign_csc(m, n, std::vector<Index>(colptr, colptr + n + 1),
+                       std::vector<Index>(rowidx, rowidx + nnz), std::vector<double>(val, val + nnz));
+  return h;
+}
+
+sankhya_model* sankhya_model_generate(int32_t kind, int32_t m, int32_t n, int32_t per_col,
+                                      double param, uint64_t seed) {
+  if (kind < 0 || kind > 2 || m <= 0 || n <= 0 || per_col < 0) return nullptr;
+  hip_engine::GenSpec g;
+  g.kind = kind;
+  g.m = m;
+  g.n = n;
+  g.per_col = per_col;
+  g.param = param;
+  g.seed = seed;
+  try {
+    auto* h = new sankhya_model();
+    h->model = hip_engine::generate(g);
+    return h;
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void sankhya_model_dims(const sankhya_model* h, int32_t* m, int32_t* n, int64_t* nnz) {
+  if (m) *m = h->model.num_rows();
+  if (n) *n = h->model.num_cols();
+  if (nnz) *nnz = h->model.num_nonzeros();
+}
+
+void sankhya_model_export(const sankhya_model* h, int32_t* colptr, int32_t* rowidx, double* val,
+                          double* cost, double* col_lower, double* col_upper, double* row_lower,
+                          double* row_upper, double* objective_offset, int32_t* maximize) {
+  const Model& md = h->model;
+  auto cp = [](auto* dst, const auto& src) {
+    if (dst) std::memcpy(dst, src.data(), sizeof(src[0]) * src.size());
+  };
+  cp(colptr, md.matrix.column_starts());
+  cp(rowidx, md.matrix.row_indices());
+  cp(val, md.matrix.values());
+  cp(cost, md.col_cost);
+  cp(col_lower, md.col_lower);
+  cp(col_upper, md.col_upper);
+  cp(row_lower, md.row_lower);
+  cp(row_upper, md.row_upper);
+  if (objective_offset) *objective_offset = md.objective_offset;
+  if (maximize) *maximize = md.sense == ObjSense::kMaximize ? 1 : 0;
+}
+
+void sankhya_model_free(sankhya_model* h) { delete h; }
+
+int sankhya_pdhg_solve(const sankhya_model* h, const char* options, double* col_value,
+                       double* row_dual, double* col_dual, double* row_activity,
+                       sankhya_result* out) {
+  Options o;
+  o.set_bool("log_to_console", false);
+  std::string error;
+  if (!apply_options(&o, options, &error)) {
+    if (out) {
+      std::memset(out, 0, sizeof(*out));
+      out->status = static_cast<int32_t>(SolveStatus::kModelError);
+      copy_text(out->message, sizeof(out->message), error);
+    }
+    return -1;
+  }
+  const Solution s = solve(h->model, o, nullptr);
+  const auto& st = hip_engine::last_engine_stats();
+  auto cp = [](double* dst, const std::vector<double>& src) {
+    if (dst && !src.empty()) std::memcpy(dst, src.data(), sizeof(double) * src.size());
+  };
+  cp(col_value, s.col_value);
+  cp(row_dual, s.row_dual);
+  cp(col_dual, s.col_dual);
+  cp(row_activity, s.row_activity);
+  if (out) {
+    std::memset(out, 0, sizeof(*out));
+    out->status = static_cast<int32_t>(s.status);
+    out->iterations = s.iterations;
+    out->attempts = st.attempts;
+    out->restarts = st.restarts;
+    out->ticks = st.ticks;
+    out->device_bytes = st.device_bytes;
+    out->objective = s.objective;
+    out->dual_bound = s.dual_bound;
+    out->solve_seconds = s.solve_seconds;
+    out->setup_seconds = st.setup_seconds;
+    out->loop_seconds = st.loop_seconds;
+    out->spectral_norm = st.spectral_norm;
+    out->primal_infeasibility = s.primal_infeasibility;
+    out->dual_infeasibility = s.dual_infeasibility;
+    out->complementarity_violation = s.complementarity_violation;
+    out->relative_gap = s.relative_gap;
+    out->final_primal = st.final_primal;
+    out->final_dual = st.final_dual;
+    out->final_gap = st.final_gap;
+    copy_text(out->algorithm, sizeof(out->algorithm), s.algorithm);
+    copy_text(out->message, sizeof(out->message), s.message);
+  }
+  return 0;
+}
+
+int sankhya_pdhg_scaling(const sankhya_model* h, int ruiz, double* row, double* column,
+                         double* scaled_val, double* cost_hat, double* col_lower_hat,
+                         double* col_upper_hat, double* row_lower_hat, double* row_upper_hat) {
+  const Model& md = h->model;
+  std::vector<double> cost(md.col_cost.size());
+  for (std::size_t j = 0; j < cost.size(); ++j) cost[j] = md.sense_multiplier() * md.col_cost[j];
+  const hip_engine::Scaling s = hip_engine::build_scaling(md, cost, ruiz);
+  auto cp = [](double* dst, const std::vector<double>& src) {
+    if (dst && !src.empty()) std::memcpy(dst, src.data(), sizeof(double) * src.size());
+  };
+  cp(row, s.row);
+  cp(column, s.column);
+  cp(scaled_val, s.values);
+  cp(cost_hat, s.cost);
+  cp(col_lower_hat, s.col_lower);
+  cp(col_upper_hat, s.col_upper);
+  cp(row_lower_hat, s.row_lower);
+  cp(row_upper_hat, s.row_upper);
+  return 0;
+}
+
+int sankhya_pdhg_device_open(const sankhya_model* h, int device, int precision, pdhg_dev** out) {
+  if (!h || !out) return PDHG_E_ARGUMENT;
+  const Model& md = h->model;
+  const Index m = md.num_rows(), n = md.num_cols();
+  std::vector<double> cost(static_cast<std::size_t>(n));
+  for (Index j = 0; j < n; ++j) cost[j] = md.sense_multiplier() * md.col_cost[j];
+  double cs = 0.0, bs = 0.0;
+  for (const double c : cost) cs += c * c;
+  for (Index i = 0; i < m; ++i) {
+    const double b = is_finite_bound(md.row_lower[i]) ? md.row_lower[i]
+                     : (is_finite_bound(md.row_upper[i]) ? md.row_upper[i] : 0.0);
+    bs += b * b;
+  }
+  const hip_engine::Scaling s = hip_engine::build_scaling(md, cost, 10);
+  // CSR of Ahat by counting transpose.
+  const auto& cp = md.matrix.column_starts();
+  const auto& ri = md.matrix.row_indices();
+  std::vector<Index> rp(static_cast<std::size_t>(m) + 1, 0), ci(s.values.size());
+  std::vector<double> cv(s.values.size());
+  for (const Index r : ri) ++rp[static_cast<std::size_t>(r) + 1];
+  for (Index i = 0; i < m; ++i) rp[i + 1] += rp[i];
+  std::vector<Index> next(rp.begin(), rp.end() - 1);
+  for (Index j = 0; j < n; ++j)
+    for (Index k = cp[j]; k < cp[j + 1]; ++k) {
+      const Index slot = next[ri[k]]++;
+      ci[slot] = j;
+      cv[slot] = s.values[k];
+    }
+  int rc = pdhg_dev_create(device, precision, out);
+  if (rc) return rc;
+  rc = pdhg_dev_upload(*out, m, n, md.num_nonzeros(), rp.data(), ci.data(), cv.data(), cp.data(),
+                       ri.data(), s.values.data(), s.cost.data(), s.col_lower.data(),
+                       s.col_upper.data(), s.row_lower.data(), s.row_upper.data(), s.column.data(),
+                       s.row.data(), cost.data(), md.col_lower.data(), md.col_upper.data(),
+                       md.row_lower.data(), md.row_upper.data(), std::sqrt(bs), std::sqrt(cs));
+  if (rc) {
+    pdhg_dev_destroy(*out);
+    *out = nullptr;
+  }
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,24 @@
+// SPDX-License-Identifier: Apache-2.0
+// Counters of the last engine run on this thread, for the C ABI and the benchmark.
+#pragma once
+
+#include <cstdint>
+
+#include "sankhya/logging.hpp"
+#include "sankhya/model.hpp"
+#include "sankhya/options.hpp"
+
+namespace sankhya::hip_engine {
+
+struct EngineStats {
+  std::int64_t iterations = 0, attempts = 0, restarts = 0, ticks = 0, device_bytes = 0;
+  double spectral_norm = 0.0;
+  double setup_seconds = 0.0, loop_seconds = 0.0, total_seconds = 0.0;
+  double final_primal = 0.0, final_dual = 0.0, final_gap = 0.0;
+};
+
+const EngineStats& last_engine_stats();
+
+Solution run(const Model& model, const Options& options, Logger& logger, SolveControl* control);
+
+}  // namespace sankhya::hip_engine

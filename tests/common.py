@@ -1,0 +1,88 @@
+"""Shared test helpers: instances, known answers, and the GPU-presence check."""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import sankhya_hip as sh
+from mps import read_mps
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NETLIB = os.path.join(HERE, "golden", "netlib")
+INF = math.inf
+
+
+def gpu_or_skip():
+    ok, desc = sh.device_description(0)
+    if not ok:
+        pytest.skip(f"no gfx950 device: {desc}")
+    return desc
+
+
+def netlib(name: str):
+    return read_mps(os.path.join(NETLIB, f"{name}.mps"))
+
+
+def netlib_optima():
+    with open(os.path.join(NETLIB, "reference_optima.json")) as fh:
+        return json.load(fh)
+
+
+def model_of(d) -> sh.Model:
+    return sh.Model.from_csc(d["m"], d["n"], d["colptr"], d["rowidx"], d["val"], d["cost"],
+                             d["col_lower"], d["col_upper"], d["row_lower"], d["row_upper"],
+                             d.get("objective_offset", 0.0), d.get("maximize", False))
+
+
+def generated(kind, m, n, per_col, seed, param=1.5):
+    mdl = sh.Model.generate(kind, m, n, per_col, seed, param)
+    return mdl, mdl.arrays()
+
+
+def c0(seed: int):
+    """The 1,000 x 3,000 uniform-equality instance, 5 nonzeros per column."""
+    return generated("uniform", 1000, 3000, 5, seed)
+
+
+def dense_lp(rows, lower, upper, cost, col_upper=None, maximize=False, offset=0.0):
+    """Same construction as make_lp in the reference's tests/unit/test_pdhg.cpp:66-89."""
+    A = np.asarray(rows, dtype=float)
+    m, n = A.shape
+    colptr, rowidx, val = [0], [], []
+    for j in range(n):
+        for i in range(m):
+            if A[i, j] != 0.0:
+                rowidx.append(i)
+                val.append(A[i, j])
+        colptr.append(len(rowidx))
+    return dict(m=m, n=n, colptr=np.array(colptr, np.int32), rowidx=np.array(rowidx, np.int32),
+                val=np.array(val), cost=np.array(cost, float), col_lower=np.zeros(n),
+                col_upper=np.array(col_upper if col_upper is not None else [INF] * n, float),
+                row_lower=np.array(lower, float), row_upper=np.array(upper, float),
+                objective_offset=offset, maximize=maximize)
+
+
+def blend_guard():
+    """reference tests/unit/test_pdhg.cpp:451-472 (make_blend_lp)."""
+    d = dense_lp([[1.0, 1.0, 1.0], [0.30, 0.45, 0.38], [0.80, -0.86, -0.22]], [90.0, 40.0, -INF],
+                 [120.0, 40.0, 0.0], [2.40, 1.60, 1.64], maximize=True)
+    d["col_lower"] = np.array([10.0, 0.0, 0.0])
+    d["col_upper"] = np.array([INF, 45.0, 60.0])
+    return d
+
+
+# (name, lp dict, expected objective, tolerance) from test_pdhg.cpp:95-146,474-488
+KNOWN_ANSWERS = [
+    ("MinimisesASum", lambda: dense_lp([[1, 1]], [2], [INF], [1, 1]), 2.0, 1e-6),
+    ("HonoursAnEqualityRow", lambda: dense_lp([[1, 1]], [4], [4], [1, 2]), 4.0, 1e-6),
+    ("HonoursARangeRow", lambda: dense_lp([[1]], [2], [6], [1]), 2.0, 1e-6),
+    ("HonoursAColumnUpperBound", lambda: dense_lp([[1]], [-INF], [100], [-1], col_upper=[7]), -7.0, 1e-6),
+    ("ReportsTheObjectiveInTheOriginalSense",
+     lambda: dense_lp([[1, 0], [0, 2], [3, 2]], [-INF] * 3, [4, 12, 18], [3, 5], maximize=True), 36.0, 1e-5),
+    ("CarriesTheObjectiveOffset", lambda: dense_lp([[1]], [3], [INF], [1], offset=10.0), 13.0, 1e-6),
+]
