@@ -1,0 +1,183 @@
+"""Benchmark: LPs solved per second by the MI355X restarted-PDHG engine, plus the roofline of
+its dominant kernel and a CPU baseline.
+
+Workload (default): one synthetic LP of the bandwidth-scale configuration, m = 200,000 rows,
+n = 500,000 columns, 10 nonzeros per column at Zipf(1.5)-distributed rows (nnz = 5,000,000),
+70% equality / 30% `<=` rows, x0 ~ U[0,1], bounds [0,10], b = A x0, c = A^T y0 + z; solved in
+plain FP64 to 1e-8 relative KKT (the reference's defaults: stop on the request AND the project's
+absolute standard). A step is one whole solve: preconditioning, upload, power iteration, the
+iterations and the report; instance generation is outside the timed region.
+
+Prints one JSON line (rank 0). One GPU only: --gpus above 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "sih-26_amd", "python"), os.path.join(ROOT, "oracle")]
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (kind, m, n, per_col, zipf exponent, seed, tolerance, stop_at_request)
+    "zipf-200k-500k-fp64": ("zipf", 200_000, 500_000, 10, 1.5, 2, 1e-8, False),
+    "uniform-1k-3k-fp64": ("uniform", 1_000, 3_000, 5, 1.5, 1, 1e-6, True),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(m, n, nnz):
+    """Algorithmic HBM bytes per launch of the two iteration kernels (FP64 values, int32
+    indices). k_dual streams the CSR of Ahat once for both right-hand sides (ext and dx come
+    from one gather of the (x, x') pair) and updates y; k_primal streams the CSR of Ahat^T and
+    updates x. Counted once per unique element: the matrix (12 nnz + 4 (rows+1)), the
+    gathered vector (16 n for the x pairs / 8 m for y), and the epilogue's per-row vectors."""
+    k_dual = 12 * nnz + 4 * (m + 1) + 16 * n + m * (8 + 16 + 8)  # y, row bounds, y' write
+    k_primal = 12 * nnz + 4 * (n + 1) + 8 * m + n * (8 + 24 + 8)  # x, cost+bounds, x' write
+    return k_primal, k_dual
+
+
+def cpu_baseline(arrays, gpu_iterations, tolerance, budget_s=20.0):
+    """The oracle (a CPU port of the reference's PDHG, one core) on a bounded sample of the
+    same LP: setup alone, then a fixed iteration budget; extrapolated to the iterations the
+    GPU needed."""
+    import oracle as orc
+
+    P = orc.Problem(arrays)
+    t = time.perf_counter()
+    r0 = orc.solve(P, tolerance=tolerance, iteration_limit=0)
+    setup = time.perf_counter() - t
+    iters = 50
+    t = time.perf_counter()
+    orc.solve(P, tolerance=tolerance, iteration_limit=iters)
+    per_iter = max((time.perf_counter() - t - setup) / iters, 1e-12)
+    if budget_s > setup + per_iter * iters * 3 and per_iter * 200 < budget_s:
+        iters2 = int(min(400, (budget_s - setup) / per_iter))
+        t = time.perf_counter()
+        orc.solve(P, tolerance=tolerance, iteration_limit=iters2)
+        per_iter = max((time.perf_counter() - t - setup) / iters2, 1e-12)
+        iters = iters2
+    est = setup + per_iter * gpu_iterations
+    return {"value": 1.0 / est, "unit": "LPs/s", "cores": 1, "kind": "port",
+            "sample": f"oracle on the same LP: setup {setup:.2f}s (scaling + power iteration) "
+                      f"timed alone, then {iters} iterations at {per_iter*1e3:.1f} ms/iteration, "
+                      f"extrapolated to the GPU's {gpu_iterations} iterations "
+                      f"({est:.1f}s per LP); status of the sample run {r0['status']}"}
+
+
+def host_info():
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except Exception:
+        model = "unknown"
+    return model, os.cpu_count()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="zipf-200k-500k-fp64", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timing-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is a design note, not part of this build)"}))
+        sys.exit(2)
+
+    import sankhya_hip as sh
+
+    ok, desc = sh.device_description(0)
+    if not ok:
+        raise SystemExit(f"bench needs a gfx950 device: {desc}")
+    kind, m, n, per_col, zexp, seed, tol, stop_req = WORKLOADS[args.workload]
+    log(f"[bench] device {desc}; generating {args.workload} (seed {seed})")
+    t = time.perf_counter()
+    model = sh.Model.generate(kind, m, n, per_col, seed, zexp)
+    M, N, NNZ = model.dims
+    log(f"[bench] generated m={M} n={N} nnz={NNZ} in {time.perf_counter()-t:.1f}s")
+    opts = dict(pdhg_tolerance=tol, pdhg_stop_at_request=stop_req, iteration_limit=1_000_000,
+                pdhg_precision=args.precision)
+
+    for w in range(args.warmup):
+        s = sh.solve(model, **opts)
+        log(f"[bench] warmup {w}: {s.status} obj={s.objective:.12g} it={s.iterations} "
+            f"t={s.solve_seconds:.3f}s")
+    times, sols = [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        s = sh.solve(model, **opts)
+        times.append(time.perf_counter() - ts)
+        sols.append(s)
+        log(f"[bench] step {k}: {s.status} obj={s.objective:.12g} it={s.iterations} "
+            f"restarts={s.stats['restarts']} t={times[-1]:.3f}s (setup {s.stats['setup_seconds']:.3f}s, "
+            f"loop {s.stats['loop_seconds']:.3f}s)")
+    total = time.perf_counter() - t0
+    solved = sum(1 for s in sols if s.status in ("optimal", "feasible"))
+    value = solved / total if total > 0 else 0.0
+    last = sols[-1]
+
+    # Roofline of the dominant kernel: standalone launches on the engine's stream, timed with
+    # HIP events, on the same uploaded LP.
+    dev = sh.Device(model, precision=args.precision)
+    dev.start(tolerance=tol, spectral_norm=last.stats["spectral_norm"])
+    dev.run(1)
+    p_ms, d_ms = dev.time_kernels(args.timing_reps)
+    dev.close()
+    bp, bd = kernel_bytes(M, N, NNZ)
+    if args.precision == "mixed":
+        bp -= 4 * NNZ + 4 * N + 4 * M
+        bd -= 4 * NNZ + 8 * N + 4 * M
+    kp = {"name": "k_primal", "ms": p_ms, "bytes": bp, "gbs": bp / (p_ms * 1e6)}
+    kd = {"name": "k_dual", "ms": d_ms, "bytes": bd, "gbs": bd / (d_ms * 1e6)}
+    dom = kd if d_ms >= p_ms else kp
+    log(f"[bench] k_primal {p_ms*1e3:.1f} us {kp['gbs']:.0f} GB/s; k_dual {d_ms*1e3:.1f} us {kd['gbs']:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] timing the CPU baseline (oracle, one core) on a bounded sample")
+        cpu = cpu_baseline(model.arrays(), last.iterations, tol)
+        cpu_model, ncpu = host_info()
+        cpu["sample"] += f"; host {cpu_model}, {ncpu} logical CPUs"
+
+    line = {
+        "metric": "LPs solved/sec to the configured relative KKT tolerance (plain FP64 GPU PDHG)",
+        "value": value,
+        "unit": "LPs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic (deterministic generator, seeded)",
+        "config": {"workload": args.workload, "m": M, "n": N, "nnz": NNZ,
+                   "tolerance": tol, "stop_at_request": stop_req, "precision": args.precision,
+                   "iterations": last.iterations, "status": last.status,
+                   "objective": last.objective, "restarts": last.stats["restarts"]},
+        "roofline": {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom["name"],
+                     "kernel_us": dom["ms"] * 1e3, "bytes_per_launch": dom["bytes"],
+                     "other_kernel": {k: v for k, v in (kp if dom is kd else kd).items()}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

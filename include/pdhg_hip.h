@@ -82,6 +82,9 @@ typedef struct {
   int32_t stop_at_request; /* pdhg_stop_at_request */
   double spectral_norm;    /* eta0 = 1/norm, ceiling = 1e3/max(norm,1e-12) */
   int32_t chunk_attempts;  /* attempt slots per captured graph; 0 = default */
+  int32_t stop_rule;       /* 0: the reference's test (request + absolute primal, and the
+                              project standard unless stop_at_request); 1: relative KKT
+                              error (max of relative primal, dual, gap) <= tolerance */
 } pdhg_params;
 
 /* Engine state as seen from the host after a sync. */

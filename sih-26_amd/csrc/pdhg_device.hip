@@ -52,7 +52,7 @@ struct Ctl {
   int64_t iteration, averaged, last_restart, restarts, iteration_limit, attempts, ntick;
   int64_t eval_count;
   int32_t cur, pending, tick_pending, halt, tick, apply, copy_best, chose_avg, restart_now;
-  int32_t done, limit_hit, use_restarts, stop_at_request, no_info, pw_zero, pad;
+  int32_t done, limit_hit, use_restarts, stop_at_request, no_info, pw_zero, stop_rule;
 };
 
 template <typename X>
@@ -78,23 +78,26 @@ __device__ __forceinline__ void load_pair(const X* buf, int64_t j, int cur, doub
   }
 }
 
-// ---- iteration kernels ------------------------------------------------------------------
+// ---- epilogues and gathers, shared by the per-item kernels and the tail kernels ----------
 
-template <typename V, typename X>
-__global__ __launch_bounds__(kThreads) void k_primal(PlanView<V> T, Iter<X> it, Ctl* ctl,
-                                                     double* part, double* lrpart, int force) {
-  __shared__ double lds[kTile];
-  __shared__ double red[4 * 2];
-  if (!force && ctl->halt) return;
-  const int cur = ctl->cur, pend = ctl->pending;
-  const double omega = ctl->omega;
-  const double tau = ctl->eta / omega;
-  auto gather = [&](int i) {
+template <typename X>
+struct PrimalGather {  // y_cur[i]
+  const X* ybuf;
+  int cur;
+  __device__ RowSum<1> operator()(int i) const {
     RowSum<1> g;
-    g.v[0] = static_cast<double>(it.ybuf[2 * (int64_t)i + cur]);
+    g.v[0] = static_cast<double>(ybuf[2 * (int64_t)i + cur]);
     return g;
-  };
-  auto epi = [&](int j, const RowSum<1>& s, Acc<1, 0>& a) {
+  }
+};
+
+// x' = proj(x - tau (c + Ahat^T y)); movement term (src/pdhg/pdhg.cpp:225-236,252-256).
+template <typename X>
+struct PrimalEpi {
+  Iter<X> it;
+  int cur, pend;
+  double tau, omega;
+  __device__ void operator()(int j, const RowSum<1>& s, Acc<1, 0>& a) const {
     const double x = static_cast<double>(it.xbuf[2 * (int64_t)j + cur]);
     if (pend) it.xsum[j] += x;
     const double grad = it.chat[j] + s.v[0];
@@ -102,36 +105,30 @@ __global__ __launch_bounds__(kThreads) void k_primal(PlanView<V> T, Iter<X> it, 
     it.xbuf[2 * (int64_t)j + (cur ^ 1)] = stored;
     const double d = static_cast<double>(stored) - x;
     a.s[0] += 0.5 * omega * d * d;
-  };
-  Acc<1, 0> acc, lacc;
-  acc.zero();
-  lacc.zero();
-  const int lr = run_item<V, 1>(T, gather, epi, acc, lacc, lds, red);
-  block_reduce(acc, red);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = acc.s[0];
-    if (lr >= 0) lrpart[lr] = lacc.s[0];
   }
-}
+};
 
-template <typename V, typename X>
-__global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ctl* ctl,
-                                                   double* part, double* lrpart, int force) {
-  __shared__ double lds[2 * kTile];
-  __shared__ double red[4 * 2];
-  if (!force && ctl->halt) return;
-  const int cur = ctl->cur, pend = ctl->pending;
-  const double omega = ctl->omega;
-  const double sigma = ctl->eta * omega;
-  auto gather = [&](int j) {
+template <typename X>
+struct DualGather {  // (ext, dx) from one (x, x') pair
+  const X* xbuf;
+  int cur;
+  __device__ RowSum<2> operator()(int j) const {
     double x, xn;
-    load_pair(it.xbuf, j, cur, x, xn);
+    load_pair(xbuf, j, cur, x, xn);
     RowSum<2> g;
     g.v[0] = 2.0 * xn - x;
     g.v[1] = xn - x;
     return g;
-  };
-  auto epi = [&](int i, const RowSum<2>& s, Acc<2, 0>& a) {
+  }
+};
+
+// Moreau prox y' = v - sigma proj(v / sigma), movement and interaction (pdhg.cpp:238-277).
+template <typename X>
+struct DualEpi {
+  Iter<X> it;
+  int cur, pend;
+  double sigma, omega;
+  __device__ void operator()(int i, const RowSum<2>& s, Acc<2, 0>& a) const {
     const double y = static_cast<double>(it.ybuf[2 * (int64_t)i + cur]);
     if (pend) it.ysum[i] += y;
     const double v = y + sigma * s.v[0];
@@ -140,73 +137,8 @@ __global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ct
     const double dy = static_cast<double>(stored) - y;
     a.s[0] += 0.5 * dy * dy / omega;
     a.s[1] += dy * s.v[1];
-  };
-  Acc<2, 0> acc, lacc;
-  acc.zero();
-  lacc.zero();
-  const int lr = run_item<V, 2>(A, gather, epi, acc, lacc, lds, red);
-  block_reduce(acc, red);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = acc.s[0];
-    part[2 * blockIdx.x + 1] = acc.s[1];
-    if (lr >= 0) {
-      lrpart[2 * lr] = lacc.s[0];
-      lrpart[2 * lr + 1] = lacc.s[1];
-    }
   }
-}
-
-// Adaptive step (src/pdhg/pdhg.cpp:293-335) and the tick test (:341-342).
-__global__ __launch_bounds__(kThreads) void k_step(Ctl* ctl, const double* pp, int np,
-                                                   const double* lrp, int nlrp,
-                                                   const double* pd, int nd, const double* lrd,
-                                                   int nlrd) {
-  __shared__ double red[4];
-  if (ctl->halt) return;
-  const double mx = block_sum_strided(pp, np, 1, 0, red, false) +
-                    block_sum_strided(lrp, nlrp, 1, 0, red, false);
-  const double my = block_sum_strided(pd, nd, 2, 0, red, false) +
-                    block_sum_strided(lrd, nlrd, 2, 0, red, false);
-  const double inter = block_sum_strided(pd, nd, 2, 1, red, false) +
-                       block_sum_strided(lrd, nlrd, 2, 1, red, false);
-  if (threadIdx.x != 0) return;
-  const double movement = mx + my;
-  const double interaction = fabs(inter);
-  const bool no_info = interaction <= 0.0;
-  const double limit = no_info ? __builtin_huge_val() : movement / interaction;
-  const int64_t iteration = ctl->iteration;
-  const double exponent = static_cast<double>(iteration + 1 > 2 ? iteration + 1 : 2);
-  const double shrink = 1.0 - pow(exponent, -0.3);
-  const double grow = 1.0 + pow(exponent, -0.6);
-  const double proposed = smin(shrink * limit, grow * ctl->eta);
-  const double eta = ctl->eta;
-  int64_t it = iteration;
-  if (eta <= limit) {
-    ctl->cur ^= 1;
-    ++it;
-    ctl->iteration = it;
-    ctl->averaged += 1;
-    ctl->pending = 1;
-  } else {
-    ctl->pending = 0;
-  }
-  if (!no_info) {
-    double e = proposed;
-    if (e < 1e-12) e = 1e-12;
-    if (ctl->eta_ceiling < e) e = ctl->eta_ceiling;
-    ctl->eta = e;
-  }
-  ctl->attempts += 1;
-  ctl->no_info = no_info ? 1 : 0;
-  const bool tick = it != 0 && (it % kEvalInterval == 0 || no_info);
-  if (tick) {
-    ctl->tick = 1;
-    ctl->halt = 1;
-  } else if (it >= ctl->iteration_limit) {
-    ctl->halt = 1;
-    ctl->limit_hit = 1;
-  }
-}
+};
 
 // ---- convergence test (src/pdhg/pdhg_evaluate.cpp:19-139), fused per row / column -------
 
@@ -247,25 +179,29 @@ __device__ __forceinline__ void col_terms(double d, double x, double cl, double 
   po += c * x;
 }
 
-// Tick: both candidates at once. Partials per workgroup (stride 9):
-//   rows: pv_c sup_c dv_c pv_a sup_a dv_a dy_restart^2 | cm_c cm_a
-//   cols: dv_c bc_c po_c dv_a bc_a po_a dx_restart^2   | cm_c cm_a
-template <typename V, typename X>
-__global__ __launch_bounds__(kThreads) void k_eval_rows(PlanView<V> A, Iter<X> it, Ctl* ctl,
-                                                        double* part, double* lrpart) {
-  __shared__ double lds[2 * kTile];
-  __shared__ double red[4 * 9];
-  if (!ctl->tick) return;
-  const int cur = ctl->cur, tp = ctl->pending;
-  const double count = static_cast<double>(ctl->averaged > 0 ? ctl->averaged : 1);
-  auto gather = [&](int j) {
+// Tick: both candidates (current, average) at once. Accumulators:
+//   rows: s = pv_c sup_c dv_c pv_a sup_a dv_a dy_restart^2 | m = cm_c cm_a
+//   cols: s = dv_c bc_c po_c dv_a bc_a po_a dx_restart^2   | m = cm_c cm_a
+template <typename X>
+struct TickRowsGather {  // x_cur[j] and the running average x_avg[j], scaled
+  Iter<X> it;
+  int cur, tp;
+  double count;
+  __device__ RowSum<2> operator()(int j) const {
     const double xc = static_cast<double>(it.xbuf[2 * (int64_t)j + cur]);
     RowSum<2> g;
     g.v[0] = xc;
     g.v[1] = (tp ? it.xsum[j] + xc : it.xsum[j]) / count;
     return g;
-  };
-  auto epi = [&](int i, const RowSum<2>& s, Acc<7, 2>& a) {
+  }
+};
+
+template <typename X>
+struct TickRowsEpi {
+  Iter<X> it;
+  int cur, tp;
+  double count;
+  __device__ void operator()(int i, const RowSum<2>& s, Acc<7, 2>& a) const {
     const double dr = it.dr[i];
     const double yh = static_cast<double>(it.ybuf[2 * (int64_t)i + cur]);
     const double yah = (tp ? it.ysum[i] + yh : it.ysum[i]) / count;
@@ -273,60 +209,265 @@ __global__ __launch_bounds__(kThreads) void k_eval_rows(PlanView<V> A, Iter<X> i
     row_terms(s.v[1] / dr, yah * dr, it.rl[i], it.ru[i], a.s[3], a.s[4], a.s[5], a.m[1]);
     const double d = yh - it.yrest[i];
     a.s[6] += d * d;
-  };
-  Acc<7, 2> acc, lacc;
-  acc.zero();
-  lacc.zero();
-  const int lr = run_item<V, 2>(A, gather, epi, acc, lacc, lds, red);
-  block_reduce(acc, red);
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < 7; ++k) part[9 * blockIdx.x + k] = acc.s[k];
-    for (int k = 0; k < 2; ++k) part[9 * blockIdx.x + 7 + k] = acc.m[k];
-    if (lr >= 0) {
-      for (int k = 0; k < 7; ++k) lrpart[9 * lr + k] = lacc.s[k];
-      for (int k = 0; k < 2; ++k) lrpart[9 * lr + 7 + k] = lacc.m[k];
-    }
   }
-}
+};
 
-template <typename V, typename X>
-__global__ __launch_bounds__(kThreads) void k_eval_cols(PlanView<V> T, Iter<X> it, Ctl* ctl,
-                                                        double* part, double* lrpart) {
-  __shared__ double lds[2 * kTile];
-  __shared__ double red[4 * 9];
-  if (!ctl->tick) return;
-  const int cur = ctl->cur, tp = ctl->pending;
-  const double count = static_cast<double>(ctl->averaged > 0 ? ctl->averaged : 1);
-  auto gather = [&](int i) {
+template <typename X>
+struct TickColsGather {
+  Iter<X> it;
+  int cur, tp;
+  double count;
+  __device__ RowSum<2> operator()(int i) const {
     const double yc = static_cast<double>(it.ybuf[2 * (int64_t)i + cur]);
     RowSum<2> g;
     g.v[0] = yc;
     g.v[1] = (tp ? it.ysum[i] + yc : it.ysum[i]) / count;
     return g;
-  };
-  auto epi = [&](int j, const RowSum<2>& s, Acc<7, 2>& a) {
+  }
+};
+
+template <typename X>
+struct TickColsEpi {
+  Iter<X> it;
+  int cur, tp;
+  double count;
+  __device__ void operator()(int j, const RowSum<2>& s, Acc<7, 2>& a) const {
     const double dc = it.dc[j];
     const double c = it.cost[j];
     const double xh = static_cast<double>(it.xbuf[2 * (int64_t)j + cur]);
     const double xah = (tp ? it.xsum[j] + xh : it.xsum[j]) / count;
     col_terms(c + s.v[0] / dc, xh * dc, it.cl[j], it.cu[j], c, a.s[0], a.s[1], a.s[2], a.m[0]);
-    col_terms(c + s.v[1] / dc, xah * dc, it.cl[j], it.cu[j], c, a.s[3], a.s[4], a.s[5],
-              a.m[1]);
+    col_terms(c + s.v[1] / dc, xah * dc, it.cl[j], it.cu[j], c, a.s[3], a.s[4], a.s[5], a.m[1]);
     const double d = xh - it.xrest[j];
     a.s[6] += d * d;
-  };
-  Acc<7, 2> acc, lacc;
+  }
+};
+
+// One unscaled point (bx, by): rows s = pv sup dv | m = cm ; cols s = dv bc po | m = cm.
+template <typename X>
+struct SingleRowsGather {
+  Iter<X> it;
+  const double* bx;
+  __device__ RowSum<1> operator()(int j) const {
+    RowSum<1> g;
+    g.v[0] = bx[j] / it.dc[j];
+    return g;
+  }
+};
+template <typename X>
+struct SingleRowsEpi {
+  Iter<X> it;
+  const double* by;
+  double* act_out;
+  __device__ void operator()(int i, const RowSum<1>& s, Acc<3, 1>& a) const {
+    const double act = s.v[0] / it.dr[i];
+    if (act_out) act_out[i] = act;
+    row_terms(act, by[i], it.rl[i], it.ru[i], a.s[0], a.s[1], a.s[2], a.m[0]);
+  }
+};
+template <typename X>
+struct SingleColsGather {
+  Iter<X> it;
+  const double* by;
+  __device__ RowSum<1> operator()(int i) const {
+    RowSum<1> g;
+    g.v[0] = by[i] / it.dr[i];
+    return g;
+  }
+};
+template <typename X>
+struct SingleColsEpi {
+  Iter<X> it;
+  const double* bx;
+  double* red_out;
+  __device__ void operator()(int j, const RowSum<1>& s, Acc<3, 1>& a) const {
+    const double d = it.cost[j] + s.v[0] / it.dc[j];
+    if (red_out) red_out[j] = d;
+    col_terms(d, bx[j], it.cl[j], it.cu[j], it.cost[j], a.s[0], a.s[1], a.s[2], a.m[0]);
+  }
+};
+
+// Plain product y = P x (optionally x / ||x|| for the power iteration), with sum of y^2.
+struct PlainGather {
+  const double* x;
+  double inv;  // 1 / length, or 0 for none
+  double len;
+  __device__ RowSum<1> operator()(int j) const {
+    RowSum<1> g;
+    g.v[0] = len > 0.0 ? x[j] / len : x[j];
+    return g;
+  }
+};
+struct PlainEpi {
+  double* y;
+  __device__ void operator()(int i, const RowSum<1>& s, Acc<1, 0>& a) const {
+    y[i] = s.v[0];
+    a.s[0] += s.v[0] * s.v[0];
+  }
+};
+
+template <typename X>
+__device__ PrimalEpi<X> primal_epi(const Iter<X>& it, const Ctl* ctl) {
+  return PrimalEpi<X>{it, ctl->cur, ctl->pending, ctl->eta / ctl->omega, ctl->omega};
+}
+template <typename X>
+__device__ DualEpi<X> dual_epi(const Iter<X>& it, const Ctl* ctl) {
+  return DualEpi<X>{it, ctl->cur, ctl->pending, ctl->eta * ctl->omega, ctl->omega};
+}
+__device__ inline double tick_count(const Ctl* ctl) {
+  return static_cast<double>(ctl->averaged > 0 ? ctl->averaged : 1);
+}
+
+// ---- iteration kernels ------------------------------------------------------------------
+
+template <typename V, typename X>
+__global__ __launch_bounds__(kThreads) void k_primal(PlanView<V> T, Iter<X> it, Ctl* ctl,
+                                                     double* part, int force) {
+  __shared__ double lds[kTile];
+  __shared__ double red[4 * 2];
+  if (!force && ctl->halt) return;
+  PrimalGather<X> gather{it.ybuf, ctl->cur};
+  PrimalEpi<X> epi = primal_epi(it, ctl);
+  Acc<1, 0> acc;
   acc.zero();
+  run_item<V, 1>(T, gather, epi, acc, lds, red);
+  block_reduce(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc.s[0];
+}
+
+// Long columns of Ahat (dense columns), completed before k_dual reads x'.
+template <typename V, typename X>
+__global__ __launch_bounds__(kTail) void k_primal_tail(PlanView<V> T, Iter<X> it, Ctl* ctl,
+                                                       double* lr_sum, int force) {
+  __shared__ double lds[kChunk];
+  __shared__ double red[16 * 1];
+  if (!force && ctl->halt) return;
+  PrimalEpi<X> epi = primal_epi(it, ctl);
+  Acc<1, 0> acc;
+  acc.zero();
+  finish_long_rows<V, 1>(T, epi, acc, lds);
+  block_reduce(acc, red);
+  if (threadIdx.x == 0) lr_sum[0] = acc.s[0];
+}
+
+template <typename V, typename X>
+__global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ctl* ctl,
+                                                   double* part, int force) {
+  __shared__ double lds[2 * kTile];
+  __shared__ double red[4 * 2];
+  if (!force && ctl->halt) return;
+  DualGather<X> gather{it.xbuf, ctl->cur};
+  DualEpi<X> epi = dual_epi(it, ctl);
+  Acc<2, 0> acc;
+  acc.zero();
+  run_item<V, 2>(A, gather, epi, acc, lds, red);
+  block_reduce(acc, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = acc.s[0];
+    part[2 * blockIdx.x + 1] = acc.s[1];
+  }
+}
+
+// Adaptive step (src/pdhg/pdhg.cpp:293-335) and the tick test (:341-342), one thread.
+__device__ void step_decide(Ctl* ctl, double mx, double my, double inter) {
+  const double movement = mx + my;
+  const double interaction = fabs(inter);
+  const bool no_info = interaction <= 0.0;
+  const double limit = no_info ? __builtin_huge_val() : movement / interaction;
+  const int64_t iteration = ctl->iteration;
+  const double exponent = static_cast<double>(iteration + 1 > 2 ? iteration + 1 : 2);
+  const double shrink = 1.0 - pow(exponent, -0.3);
+  const double grow = 1.0 + pow(exponent, -0.6);
+  const double proposed = smin(shrink * limit, grow * ctl->eta);
+  const double eta = ctl->eta;
+  int64_t it = iteration;
+  if (eta <= limit) {
+    ctl->cur ^= 1;
+    ++it;
+    ctl->iteration = it;
+    ctl->averaged += 1;
+    ctl->pending = 1;
+  } else {
+    ctl->pending = 0;
+  }
+  if (!no_info) {
+    double e = proposed;
+    if (e < 1e-12) e = 1e-12;
+    if (ctl->eta_ceiling < e) e = ctl->eta_ceiling;
+    ctl->eta = e;
+  }
+  ctl->attempts += 1;
+  ctl->no_info = no_info ? 1 : 0;
+  const bool tick = it != 0 && (it % kEvalInterval == 0 || no_info);
+  if (tick) {
+    ctl->tick = 1;
+    ctl->halt = 1;
+  } else if (it >= ctl->iteration_limit) {
+    ctl->halt = 1;
+    ctl->limit_hit = 1;
+  }
+}
+
+// Completes the long rows of Ahat, then adds every partial in index order and takes the
+// step decision: one workgroup, after k_primal(+tail) and k_dual.
+template <typename V, typename X>
+__global__ __launch_bounds__(kTail) void k_dual_tail(PlanView<V> A, Iter<X> it, Ctl* ctl,
+                                                     const double* part_p, int np,
+                                                     const double* lr_p, int has_lr_p,
+                                                     const double* part_d, int nd, int force) {
+  __shared__ double lds[2 * kChunk];
+  __shared__ double red[16 * 2];
+  if (!force && ctl->halt) return;
+  DualEpi<X> epi = dual_epi(it, ctl);
+  Acc<2, 0> lacc;
   lacc.zero();
-  const int lr = run_item<V, 2>(T, gather, epi, acc, lacc, lds, red);
+  finish_long_rows<V, 2>(A, epi, lacc, lds);
+  block_reduce(lacc, red);
+  constexpr bool kSum1[1] = {false};
+  constexpr bool kSum2[2] = {false, false};
+  double p[1], d[2];
+  block_sums<1>(part_p, np, lr_p, has_lr_p ? 1 : 0, 1, kSum1, p, red);
+  block_sums<2>(part_d, nd, nullptr, 0, 2, kSum2, d, red);
+  if (threadIdx.x == 0) step_decide(ctl, p[0], d[0] + lacc.s[0], d[1] + lacc.s[1]);
+}
+
+template <typename V, typename X>
+__global__ __launch_bounds__(kThreads) void k_eval_rows(PlanView<V> A, Iter<X> it, Ctl* ctl,
+                                                        double* part) {
+  __shared__ double lds[2 * kTile];
+  __shared__ double red[4 * 9];
+  if (!ctl->tick) return;
+  const int cur = ctl->cur, tp = ctl->pending;
+  const double count = tick_count(ctl);
+  TickRowsGather<X> gather{it, cur, tp, count};
+  TickRowsEpi<X> epi{it, cur, tp, count};
+  Acc<7, 2> acc;
+  acc.zero();
+  run_item<V, 2>(A, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) {
     for (int k = 0; k < 7; ++k) part[9 * blockIdx.x + k] = acc.s[k];
     for (int k = 0; k < 2; ++k) part[9 * blockIdx.x + 7 + k] = acc.m[k];
-    if (lr >= 0) {
-      for (int k = 0; k < 7; ++k) lrpart[9 * lr + k] = lacc.s[k];
-      for (int k = 0; k < 2; ++k) lrpart[9 * lr + 7 + k] = lacc.m[k];
-    }
+  }
+}
+
+template <typename V, typename X>
+__global__ __launch_bounds__(kThreads) void k_eval_cols(PlanView<V> T, Iter<X> it, Ctl* ctl,
+                                                        double* part) {
+  __shared__ double lds[2 * kTile];
+  __shared__ double red[4 * 9];
+  if (!ctl->tick) return;
+  const int cur = ctl->cur, tp = ctl->pending;
+  const double count = tick_count(ctl);
+  TickColsGather<X> gather{it, cur, tp, count};
+  TickColsEpi<X> epi{it, cur, tp, count};
+  Acc<7, 2> acc;
+  acc.zero();
+  run_item<V, 2>(T, gather, epi, acc, lds, red);
+  block_reduce(acc, red);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 7; ++k) part[9 * blockIdx.x + k] = acc.s[k];
+    for (int k = 0; k < 2; ++k) part[9 * blockIdx.x + 7 + k] = acc.m[k];
   }
 }
 
@@ -357,52 +498,61 @@ __device__ inline bool meets_standard(const Res& r) {
   return r.abs_p <= 1e-7 && r.abs_d <= 1e-7 && r.gap_v <= 1e-9 && r.compl_ <= 1e-6;
 }
 
-// Sums (idx < 7) or maxima (idx 7, 8) of a 9-wide partial array plus its long-row slots.
-__device__ inline double part9(const double* p, int np, const double* lp, int nl, int idx,
-                               double* red) {
-  const bool mx = idx >= 7;
-  const double a = block_sum_strided(p, np, 9, idx, red, mx);
-  const double b = block_sum_strided(lp, nl, 9, idx, red, mx);
-  return mx ? smax(a, b) : a + b;
+// Adds a long-row accumulator into 9-wide totals (7 sums, 2 maxima).
+__device__ inline void merge9(double (&v)[9], const Acc<7, 2>& a) {
+  for (int k = 0; k < 7; ++k) v[k] += a.s[k];
+  for (int k = 0; k < 2; ++k) v[7 + k] = smax(v[7 + k], a.m[k]);
 }
 
-// Best tracking, stopping test, restart and primal weight (src/pdhg/pdhg.cpp:344-483).
-__global__ __launch_bounds__(kThreads) void k_decide(Ctl* ctl, const double* per, int nper,
-                                                     const double* lper, int nlper,
-                                                     const double* pec, int npec,
-                                                     const double* lpec, int nlpec,
-                                                     pdhg_tick* ticks) {
-  __shared__ double red[4];
-  __shared__ double v[18];
+// Best tracking, stopping test, restart and primal weight (src/pdhg/pdhg.cpp:344-483), after
+// completing the long rows of both products.
+template <typename V, typename X>
+__global__ __launch_bounds__(kTail) void k_decide(PlanView<V> A, PlanView<V> T, Iter<X> it,
+                                                  Ctl* ctl, const double* per, int nper,
+                                                  const double* pec, int npec,
+                                                  pdhg_tick* ticks) {
+  __shared__ double lds[2 * kChunk];
+  __shared__ double red[16 * 9];
   if (!ctl->tick) return;
-  for (int k = 0; k < 9; ++k) {
-    const double a = part9(per, nper, lper, nlper, k, red);
-    const double b = part9(pec, npec, lpec, nlpec, k, red);
-    if (threadIdx.x == 0) {
-      v[k] = a;
-      v[9 + k] = b;
-    }
-  }
-  __syncthreads();
+  const int cur = ctl->cur, tp = ctl->pending;
+  const double count = tick_count(ctl);
+  TickRowsEpi<X> repi{it, cur, tp, count};
+  TickColsEpi<X> cepi{it, cur, tp, count};
+  Acc<7, 2> ra, ca;
+  ra.zero();
+  ca.zero();
+  finish_long_rows<V, 2>(A, repi, ra, lds);
+  finish_long_rows<V, 2>(T, cepi, ca, lds);
+  block_reduce(ra, red);
+  block_reduce(ca, red);
+  constexpr bool kMax9[9] = {false, false, false, false, false, false, false, true, true};
+  double vr[9], vc[9];
+  block_sums<9>(per, nper, nullptr, 0, 9, kMax9, vr, red);
+  block_sums<9>(pec, npec, nullptr, 0, 9, kMax9, vc, red);
   if (threadIdx.x != 0) return;
+  merge9(vr, ra);
+  merge9(vc, ca);
   // rows: 0 pv_c 1 sup_c 2 dv_c 3 pv_a 4 sup_a 5 dv_a 6 dyr 7 cm_c 8 cm_a
-  // cols: 9 dv_c 10 bc_c 11 po_c 12 dv_a 13 bc_a 14 po_a 15 dxr 16 cm_c 17 cm_a
+  // cols: 0 dv_c 1 bc_c 2 po_c 3 dv_a 4 bc_a 5 po_a 6 dxr 7 cm_c 8 cm_a
   const double bn = ctl->bound_norm, cn = ctl->cost_norm;
-  const Res cur = make_res(v[0], v[9] + v[2], smax(v[7], v[16]), v[11], v[10] - v[1], bn, cn);
-  Res better = cur;
+  const Res rc = make_res(vr[0], vc[0] + vr[2], smax(vr[7], vc[7]), vc[2], vc[1] - vr[1], bn, cn);
+  Res better = rc;
   int chose_avg = 0;
   if (ctl->averaged > 0) {
-    const Res avg =
-        make_res(v[3], v[12] + v[5], smax(v[8], v[17]), v[14], v[13] - v[4], bn, cn);
-    if (worst(avg) < worst(cur)) {
-      better = avg;
+    const Res ravg = make_res(vr[3], vc[3] + vr[5], smax(vr[8], vc[8]), vc[5], vc[4] - vr[4], bn, cn);
+    if (worst(ravg) < worst(rc)) {
+      better = ravg;
       chose_avg = 1;
     }
   }
   int copy = 0;
   if (worst(better) < ctl->best_worst) copy = 1;
-  const bool stop = meets_request(better, ctl->tolerance) &&
-                    (ctl->stop_at_request || meets_standard(better));
+  // The reference's test (src/pdhg/pdhg.cpp:424-425), or, when asked for, the relative KKT
+  // error alone (max of the relative primal, dual and gap residuals).
+  const bool stop = ctl->stop_rule == 1
+                        ? worst(better) <= ctl->tolerance
+                        : meets_request(better, ctl->tolerance) &&
+                              (ctl->stop_at_request || meets_standard(better));
   int restarted = 0;
   if (stop) {
     copy = 1;
@@ -414,7 +564,7 @@ __global__ __launch_bounds__(kThreads) void k_decide(Ctl* ctl, const double* per
     int64_t art = static_cast<int64_t>(0.36 * static_cast<double>(ctl->iteration));
     if (art < kEvalInterval) art = kEvalInterval;
     if (sufficient || since >= art) {
-      const double dxn = sqrt(v[15]), dyn = sqrt(v[6]);
+      const double dxn = sqrt(vc[6]), dyn = sqrt(vr[6]);
       if (dxn > 1e-12 && dyn > 1e-12) {
         double om = exp(0.5 * log(dyn / dxn) + (1.0 - 0.5) * log(ctl->omega));
         if (om < 1e-6) om = 1e-6;
@@ -523,92 +673,72 @@ __global__ void k_init(Iter<X> it) {
   }
 }
 
-// Convergence test of one unscaled point (bx, by). Partials stride 4: 3 sums + 1 max.
 template <typename V, typename X>
 __global__ __launch_bounds__(kThreads) void k_single_rows(PlanView<V> A, Iter<X> it,
                                                           const double* bx, const double* by,
-                                                          double* act_out, double* part,
-                                                          double* lrpart) {
+                                                          double* act_out, double* part) {
   __shared__ double lds[kTile];
   __shared__ double red[4 * 4];
-  auto gather = [&](int j) {
-    RowSum<1> g;
-    g.v[0] = bx[j] / it.dc[j];
-    return g;
-  };
-  auto epi = [&](int i, const RowSum<1>& s, Acc<3, 1>& a) {
-    const double act = s.v[0] / it.dr[i];
-    if (act_out) act_out[i] = act;
-    row_terms(act, by[i], it.rl[i], it.ru[i], a.s[0], a.s[1], a.s[2], a.m[0]);
-  };
-  Acc<3, 1> acc, lacc;
+  SingleRowsGather<X> gather{it, bx};
+  SingleRowsEpi<X> epi{it, by, act_out};
+  Acc<3, 1> acc;
   acc.zero();
-  lacc.zero();
-  const int lr = run_item<V, 1>(A, gather, epi, acc, lacc, lds, red);
+  run_item<V, 1>(A, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) {
     for (int k = 0; k < 3; ++k) part[4 * blockIdx.x + k] = acc.s[k];
     part[4 * blockIdx.x + 3] = acc.m[0];
-    if (lr >= 0) {
-      for (int k = 0; k < 3; ++k) lrpart[4 * lr + k] = lacc.s[k];
-      lrpart[4 * lr + 3] = lacc.m[0];
-    }
   }
 }
 
 template <typename V, typename X>
 __global__ __launch_bounds__(kThreads) void k_single_cols(PlanView<V> T, Iter<X> it,
                                                           const double* bx, const double* by,
-                                                          double* red_out, double* part,
-                                                          double* lrpart) {
+                                                          double* red_out, double* part) {
   __shared__ double lds[kTile];
   __shared__ double red[4 * 4];
-  auto gather = [&](int i) {
-    RowSum<1> g;
-    g.v[0] = by[i] / it.dr[i];
-    return g;
-  };
-  auto epi = [&](int j, const RowSum<1>& s, Acc<3, 1>& a) {
-    const double d = it.cost[j] + s.v[0] / it.dc[j];
-    if (red_out) red_out[j] = d;
-    col_terms(d, bx[j], it.cl[j], it.cu[j], it.cost[j], a.s[0], a.s[1], a.s[2], a.m[0]);
-  };
-  Acc<3, 1> acc, lacc;
+  SingleColsGather<X> gather{it, by};
+  SingleColsEpi<X> epi{it, bx, red_out};
+  Acc<3, 1> acc;
   acc.zero();
-  lacc.zero();
-  const int lr = run_item<V, 1>(T, gather, epi, acc, lacc, lds, red);
+  run_item<V, 1>(T, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) {
     for (int k = 0; k < 3; ++k) part[4 * blockIdx.x + k] = acc.s[k];
     part[4 * blockIdx.x + 3] = acc.m[0];
-    if (lr >= 0) {
-      for (int k = 0; k < 3; ++k) lrpart[4 * lr + k] = lacc.s[k];
-      lrpart[4 * lr + 3] = lacc.m[0];
-    }
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_single_decide(Ctl* ctl, const double* pr, int npr,
-                                                            const double* lpr, int nlpr,
-                                                            const double* pc, int npc,
-                                                            const double* lpc, int nlpc) {
-  __shared__ double red[4];
-  __shared__ double v[8];
-  for (int k = 0; k < 4; ++k) {
-    const bool mx = k == 3;
-    const double a = block_sum_strided(pr, npr, 4, k, red, mx);
-    const double b = block_sum_strided(lpr, nlpr, 4, k, red, mx);
-    const double c = block_sum_strided(pc, npc, 4, k, red, mx);
-    const double d = block_sum_strided(lpc, nlpc, 4, k, red, mx);
-    if (threadIdx.x == 0) {
-      v[k] = mx ? smax(a, b) : a + b;
-      v[4 + k] = mx ? smax(c, d) : c + d;
-    }
-  }
-  __syncthreads();
+template <typename V, typename X>
+__global__ __launch_bounds__(kTail) void k_single_decide(PlanView<V> A, PlanView<V> T, Iter<X> it,
+                                                         Ctl* ctl, const double* bx,
+                                                         const double* by, double* act_out,
+                                                         double* red_out, const double* pr,
+                                                         int npr, const double* pc, int npc) {
+  __shared__ double lds[kChunk];
+  __shared__ double red[16 * 4];
+  SingleRowsEpi<X> repi{it, by, act_out};
+  SingleColsEpi<X> cepi{it, bx, red_out};
+  Acc<3, 1> ra, ca;
+  ra.zero();
+  ca.zero();
+  finish_long_rows<V, 1>(A, repi, ra, lds);
+  finish_long_rows<V, 1>(T, cepi, ca, lds);
+  block_reduce(ra, red);
+  block_reduce(ca, red);
+  constexpr bool kMax4[4] = {false, false, false, true};
+  double vr[4], vc[4];
+  block_sums<4>(pr, npr, nullptr, 0, 4, kMax4, vr, red);
+  block_sums<4>(pc, npc, nullptr, 0, 4, kMax4, vc, red);
   if (threadIdx.x != 0) return;
-  // rows: 0 pv 1 sup 2 dv 3 cm ; cols: 4 dv 5 bc 6 po 7 cm
-  const Res r = make_res(v[0], v[4] + v[2], smax(v[3], v[7]), v[6], v[5] - v[1],
+  for (int k = 0; k < 3; ++k) {
+    vr[k] += ra.s[k];
+    vc[k] += ca.s[k];
+  }
+  vr[3] = smax(vr[3], ra.m[0]);
+  vc[3] = smax(vc[3], ca.m[0]);
+  // rows: 0 pv 1 sup 2 dv 3 cm ; cols: 0 dv 1 bc 2 po 3 cm
+  const Res r = make_res(vr[0], vc[0] + vr[2], smax(vr[3], vc[3]), vc[2], vc[1] - vr[1],
                          ctl->bound_norm, ctl->cost_norm);
   ctl->final_res[0] = r.primal;
   ctl->final_res[1] = r.dual;
@@ -621,44 +751,48 @@ __global__ __launch_bounds__(kThreads) void k_single_decide(Ctl* ctl, const doub
   ctl->final_res[8] = r.compl_;
 }
 
-// Plain products for the test hook and the power iteration.
+// Plain products for the test hook and the power iteration. With `scale` set, the input is
+// x / ctl->pw_len (the power iteration's normalisation, src/la/scaling.cpp:180).
 template <typename V>
 __global__ __launch_bounds__(kThreads) void k_spmv(PlanView<V> P, const double* x, double* y,
-                                                   const Ctl* scale_by_len, double* part,
-                                                   double* lrpart) {
+                                                   const Ctl* scale, double* part) {
   __shared__ double lds[kTile];
   __shared__ double red[4];
-  const double len = scale_by_len ? scale_by_len->pw_len : 1.0;
-  if (scale_by_len && scale_by_len->pw_zero) return;
-  auto gather = [&](int j) {
-    RowSum<1> g;
-    g.v[0] = scale_by_len ? x[j] / len : x[j];
-    return g;
-  };
-  auto epi = [&](int i, const RowSum<1>& s, Acc<1, 0>& a) {
-    y[i] = s.v[0];
-    a.s[0] += s.v[0] * s.v[0];
-  };
-  Acc<1, 0> acc, lacc;
+  if (scale && scale->pw_zero) return;
+  PlainGather gather{x, 0.0, scale ? scale->pw_len : 0.0};
+  PlainEpi epi{y};
+  Acc<1, 0> acc;
   acc.zero();
-  lacc.zero();
-  const int lr = run_item<V, 1>(P, gather, epi, acc, lacc, lds, red);
+  run_item<V, 1>(P, gather, epi, acc, lds, red);
   block_reduce(acc, red);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = acc.s[0];
-    if (lr >= 0) lrpart[lr] = lacc.s[0];
-  }
+  if (threadIdx.x == 0) part[blockIdx.x] = acc.s[0];
+}
+
+template <typename V>
+__global__ __launch_bounds__(kTail) void k_spmv_tail(PlanView<V> P, double* y, const Ctl* scale,
+                                                     double* lr_sum) {
+  __shared__ double lds[kChunk];
+  __shared__ double red[16];
+  if (scale && scale->pw_zero) return;
+  PlainEpi epi{y};
+  Acc<1, 0> acc;
+  acc.zero();
+  finish_long_rows<V, 1>(P, epi, acc, lds);
+  block_reduce(acc, red);
+  if (threadIdx.x == 0) lr_sum[0] = acc.s[0];
 }
 
 // Power-iteration step end (src/la/scaling.cpp:185-187): next = ||v||^2, norm = next^(1/4);
 // the next step divides by length = sqrt(next).
 __global__ __launch_bounds__(kThreads) void k_pw_finish(Ctl* ctl, const double* p, int np,
-                                                        const double* lp, int nlp) {
+                                                        const double* lr_sum) {
   __shared__ double red[4];
   if (ctl->pw_zero) return;
-  const double next = block_sum_strided(p, np, 1, 0, red, false) +
-                      block_sum_strided(lp, nlp, 1, 0, red, false);
+  constexpr bool kSum1[1] = {false};
+  double v[1];
+  block_sums<1>(p, np, lr_sum, 1, 1, kSum1, v, red);
   if (threadIdx.x != 0) return;
+  const double next = v[0];
   ctl->pw_norm = sqrt(sqrt(next));
   ctl->pw_len = sqrt(next);
   if (next == 0.0) ctl->pw_zero = 1;
@@ -674,8 +808,10 @@ struct DevPlan {
   int4* items = nullptr;
   int32_t* lr_nseg = nullptr;
   int32_t* lr_base = nullptr;
+  int32_t* lr_row = nullptr;
+  int2* lr_groups = nullptr;
+  int32_t ngroups = 0;
   double* seg_part = nullptr;
-  int32_t* lr_count = nullptr;
   template <typename V>
   PlanView<V> view() const {
     PlanView<V> p;
@@ -685,17 +821,20 @@ struct DevPlan {
     p.items = items;
     p.lr_nseg = lr_nseg;
     p.lr_base = lr_base;
+    p.lr_row = lr_row;
+    p.lr_groups = lr_groups;
     p.seg_part = seg_part;
-    p.lr_count = lr_count;
     p.nitems = nitems;
     p.nlr = nlr;
+    p.ngroups = ngroups;
     return p;
   }
 };
 
 struct HostPlan {
   std::vector<int4> items;
-  std::vector<int32_t> lr_nseg, lr_base;
+  std::vector<int32_t> lr_nseg, lr_base, lr_row;
+  std::vector<int2> groups;  // consecutive long rows whose segments fit one LDS round
   int32_t nseg = 0;
 };
 
@@ -711,6 +850,7 @@ HostPlan build_plan(int32_t rows, const int32_t* rowptr) {
       const int32_t nseg = (len + kSeg - 1) / kSeg;
       hp.lr_nseg.push_back(nseg);
       hp.lr_base.push_back(hp.nseg);
+      hp.lr_row.push_back(r);
       for (int32_t s = 0; s < nseg; ++s) {
         const int32_t k0 = rowptr[r] + s * kSeg;
         const int32_t k1 = std::min(rowptr[r] + (s + 1) * kSeg, rowptr[r + 1]);
@@ -729,6 +869,13 @@ HostPlan build_plan(int32_t rows, const int32_t* rowptr) {
       ++r;
     }
     hp.items.push_back(make_int4(r0, r, rowptr[r0], rowptr[r]));
+  }
+  const int32_t nlr = static_cast<int32_t>(hp.lr_nseg.size());
+  for (int32_t lr = 0; lr < nlr;) {
+    int32_t end = lr + 1;
+    while (end < nlr && hp.lr_base[end] + hp.lr_nseg[end] - hp.lr_base[lr] <= kChunk) ++end;
+    hp.groups.push_back(make_int2(lr, end));
+    lr = end;
   }
   return hp;
 }
@@ -754,8 +901,8 @@ struct pdhg_dev {
   double *chat = nullptr, *lhat = nullptr, *uhat = nullptr, *rlhat = nullptr, *ruhat = nullptr;
   double *dc = nullptr, *dr = nullptr, *cost = nullptr, *cl = nullptr, *cu = nullptr,
          *rl = nullptr, *ru = nullptr;
-  double *part_p = nullptr, *lr_p = nullptr, *part_d = nullptr, *lr_d = nullptr;
-  double *part_er = nullptr, *lr_er = nullptr, *part_ec = nullptr, *lr_ec = nullptr;
+  double *part_p = nullptr, *lr_p = nullptr, *part_d = nullptr;
+  double *part_er = nullptr, *part_ec = nullptr, *lr_sum = nullptr;
   Ctl* ctl = nullptr;
   Ctl* ctl_host = nullptr;
   pdhg_tick* ticks = nullptr;
@@ -830,6 +977,9 @@ bool plan_ok(const HostPlan& hp, int32_t rows, const int32_t* rowptr) {
       }
     }
   }
+  // A single long row must fit one LDS round of the tail kernels (kChunk segment sums).
+  for (const int32_t ns : hp.lr_nseg)
+    if (ns > kChunk) return false;
   return next_row == rows;
 }
 
@@ -859,9 +1009,10 @@ int make_plan(pdhg_dev* h, DevPlan* P, int32_t rows, int64_t nnz, const int32_t*
   if ((rc = upload(h, &P->items, hp.items.data(), hp.items.size()))) return rc;
   if ((rc = upload(h, &P->lr_nseg, hp.lr_nseg.data(), hp.lr_nseg.size()))) return rc;
   if ((rc = upload(h, &P->lr_base, hp.lr_base.data(), hp.lr_base.size()))) return rc;
+  if ((rc = upload(h, &P->lr_row, hp.lr_row.data(), hp.lr_row.size()))) return rc;
+  if ((rc = upload(h, &P->lr_groups, hp.groups.data(), hp.groups.size()))) return rc;
+  P->ngroups = static_cast<int32_t>(hp.groups.size());
   if ((rc = dalloc(h, &P->seg_part, static_cast<size_t>(hp.nseg) * 2))) return rc;
-  if ((rc = dalloc(h, &P->lr_count, static_cast<size_t>(P->nlr)))) return rc;
-  HIP_TRY(h, hipMemsetAsync(P->lr_count, 0, sizeof(int32_t) * std::max<int32_t>(P->nlr, 1), h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return PDHG_OK;
 }
@@ -904,28 +1055,27 @@ void launch_attempt(pdhg_dev* h, int force) {
   const Iter<X> it = iter_of<X>(h);
   if (h->T.nitems > 0)
     k_primal<V, X><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<V>(), it, h->ctl, h->part_p,
-                                                            h->lr_p, force);
+                                                            force);
+  if (h->T.nlr > 0)
+    k_primal_tail<V, X><<<1, kTail, 0, h->stream>>>(h->T.view<V>(), it, h->ctl, h->lr_p, force);
   if (h->A.nitems > 0)
     k_dual<V, X><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<V>(), it, h->ctl, h->part_d,
-                                                          h->lr_d, force);
+                                                          force);
+  k_dual_tail<V, X><<<1, kTail, 0, h->stream>>>(h->A.view<V>(), it, h->ctl, h->part_p, h->T.nitems,
+                                                h->lr_p, h->T.nlr > 0 ? 1 : 0, h->part_d,
+                                                h->A.nitems, force);
 }
 
 template <typename V, typename X>
 void launch_chunk(pdhg_dev* h) {
   const Iter<X> it = iter_of<X>(h);
-  for (int s = 0; s < h->chunk; ++s) {
-    launch_attempt<V, X>(h, 0);
-    k_step<<<1, kThreads, 0, h->stream>>>(h->ctl, h->part_p, h->T.nitems, h->lr_p, h->T.nlr,
-                                          h->part_d, h->A.nitems, h->lr_d, h->A.nlr);
-  }
+  for (int s = 0; s < h->chunk; ++s) launch_attempt<V, X>(h, 0);
   if (h->A.nitems > 0)
-    k_eval_rows<V, X><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<V>(), it, h->ctl,
-                                                               h->part_er, h->lr_er);
+    k_eval_rows<V, X><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<V>(), it, h->ctl, h->part_er);
   if (h->T.nitems > 0)
-    k_eval_cols<V, X><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<V>(), it, h->ctl,
-                                                               h->part_ec, h->lr_ec);
-  k_decide<<<1, kThreads, 0, h->stream>>>(h->ctl, h->part_er, h->A.nitems, h->lr_er, h->A.nlr,
-                                          h->part_ec, h->T.nitems, h->lr_ec, h->T.nlr, h->ticks);
+    k_eval_cols<V, X><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<V>(), it, h->ctl, h->part_ec);
+  k_decide<V, X><<<1, kTail, 0, h->stream>>>(h->A.view<V>(), h->T.view<V>(), it, h->ctl, h->part_er,
+                                             h->A.nitems, h->part_ec, h->T.nitems, h->ticks);
   k_apply<X><<<apply_grid(h), kThreads, 0, h->stream>>>(it, h->ctl);
   k_clear<<<1, 64, 0, h->stream>>>(h->ctl);
 }
@@ -935,13 +1085,23 @@ void launch_single(pdhg_dev* h, const double* bx, const double* by, double* act,
   const Iter<X> it = iter_of<X>(h);
   if (h->A.nitems > 0)
     k_single_rows<V, X><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<V>(), it, bx, by, act,
-                                                                 h->part_er, h->lr_er);
+                                                                 h->part_er);
   if (h->T.nitems > 0)
     k_single_cols<V, X><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<V>(), it, bx, by, red,
-                                                                 h->part_ec, h->lr_ec);
-  k_single_decide<<<1, kThreads, 0, h->stream>>>(h->ctl, h->part_er, h->A.nitems, h->lr_er,
-                                                 h->A.nlr, h->part_ec, h->T.nitems, h->lr_ec,
-                                                 h->T.nlr);
+                                                                 h->part_ec);
+  k_single_decide<V, X><<<1, kTail, 0, h->stream>>>(h->A.view<V>(), h->T.view<V>(), it, h->ctl, bx,
+                                                    by, act, red, h->part_er, h->A.nitems,
+                                                    h->part_ec, h->T.nitems);
+}
+
+// y = P x through the same work items; long rows completed by the tail kernel. With `scale`
+// the input is divided by the power iteration's current length.
+template <typename V>
+void launch_spmv(pdhg_dev* h, const DevPlan& P, const double* x, double* y, const Ctl* scale,
+                 double* part, double* lr_sum) {
+  if (P.nitems > 0)
+    k_spmv<V><<<P.nitems, kThreads, 0, h->stream>>>(P.view<V>(), x, y, scale, part);
+  k_spmv_tail<V><<<1, kTail, 0, h->stream>>>(P.view<V>(), y, scale, lr_sum);
 }
 
 bool mixed(const pdhg_dev* h) { return h->precision == PDHG_PRECISION_MIXED; }
@@ -1072,12 +1232,9 @@ int pdhg_dev_upload(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz, const int32_
     return rc;
   const size_t ia = static_cast<size_t>(std::max(h->A.nitems, 1));
   const size_t it = static_cast<size_t>(std::max(h->T.nitems, 1));
-  const size_t la = static_cast<size_t>(std::max(h->A.nlr, 1));
-  const size_t lt = static_cast<size_t>(std::max(h->T.nlr, 1));
-  if ((rc = dalloc(h, &h->part_p, it)) || (rc = dalloc(h, &h->lr_p, lt)) ||
-      (rc = dalloc(h, &h->part_d, 2 * ia)) || (rc = dalloc(h, &h->lr_d, 2 * la)) ||
-      (rc = dalloc(h, &h->part_er, 9 * ia)) || (rc = dalloc(h, &h->lr_er, 9 * la)) ||
-      (rc = dalloc(h, &h->part_ec, 9 * it)) || (rc = dalloc(h, &h->lr_ec, 9 * lt)) ||
+  if ((rc = dalloc(h, &h->part_p, it)) || (rc = dalloc(h, &h->lr_p, 1)) ||
+      (rc = dalloc(h, &h->part_d, 2 * ia)) || (rc = dalloc(h, &h->lr_sum, 1)) ||
+      (rc = dalloc(h, &h->part_er, 9 * ia)) || (rc = dalloc(h, &h->part_ec, 9 * it)) ||
       (rc = dalloc(h, &h->ctl, 1)) || (rc = dalloc(h, &h->ticks, static_cast<size_t>(kTickCap))))
     return rc;
   Ctl c;
@@ -1101,15 +1258,10 @@ int pdhg_dev_spmv(pdhg_dev* h, int transpose, const double* x, double* y) {
   double* din = transpose ? h->sy : h->sx;
   double* dout = transpose ? h->sx2 : h->sy;
   HIP_TRY(h, hipMemcpyAsync(din, x, sizeof(double) * in, hipMemcpyHostToDevice, h->stream));
-  if (P.nitems > 0) {
-    double* part = transpose ? h->part_ec : h->part_er;
-    double* lrp = transpose ? h->lr_ec : h->lr_er;
-    if (mixed(h))
-      k_spmv<float><<<P.nitems, kThreads, 0, h->stream>>>(P.view<float>(), din, dout, nullptr, part, lrp);
-    else
-      k_spmv<double><<<P.nitems, kThreads, 0, h->stream>>>(P.view<double>(), din, dout, nullptr, part, lrp);
-    HIP_TRY(h, hipGetLastError());
-  }
+  double* part = transpose ? h->part_ec : h->part_er;
+  if (mixed(h)) launch_spmv<float>(h, P, din, dout, nullptr, part, h->lr_sum);
+  else launch_spmv<double>(h, P, din, dout, nullptr, part, h->lr_sum);
+  HIP_TRY(h, hipGetLastError());
   HIP_TRY(h, hipMemcpyAsync(y, dout, sizeof(double) * out, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return PDHG_OK;
@@ -1141,13 +1293,13 @@ int pdhg_dev_spectral_norm(pdhg_dev* h, const double* v0, int iterations, double
   HIP_TRY(h, hipMemcpyAsync(h->sx, v0, sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
   for (int step = 0; step < iterations; ++step) {
     if (mixed(h)) {
-      k_spmv<float><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<float>(), h->sx, h->sy, h->ctl, h->part_er, h->lr_er);
-      k_spmv<float><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<float>(), h->sy, h->sx, nullptr, h->part_ec, h->lr_ec);
+      launch_spmv<float>(h, h->A, h->sx, h->sy, h->ctl, h->part_er, h->lr_sum);
+      launch_spmv<float>(h, h->T, h->sy, h->sx, nullptr, h->part_ec, h->lr_sum);
     } else {
-      k_spmv<double><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<double>(), h->sx, h->sy, h->ctl, h->part_er, h->lr_er);
-      k_spmv<double><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<double>(), h->sy, h->sx, nullptr, h->part_ec, h->lr_ec);
+      launch_spmv<double>(h, h->A, h->sx, h->sy, h->ctl, h->part_er, h->lr_sum);
+      launch_spmv<double>(h, h->T, h->sy, h->sx, nullptr, h->part_ec, h->lr_sum);
     }
-    k_pw_finish<<<1, kThreads, 0, h->stream>>>(h->ctl, h->part_ec, h->T.nitems, h->lr_ec, h->T.nlr);
+    k_pw_finish<<<1, kThreads, 0, h->stream>>>(h->ctl, h->part_ec, h->T.nitems, h->lr_sum);
     HIP_TRY(h, hipGetLastError());
   }
   HIP_TRY(h, hipMemcpyAsync(&c, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
@@ -1183,6 +1335,7 @@ int pdhg_dev_start(pdhg_dev* h, const pdhg_params* p) {
   c.iteration_limit = p->iteration_limit;
   c.use_restarts = p->restart ? 1 : 0;
   c.stop_at_request = p->stop_at_request ? 1 : 0;
+  c.stop_rule = p->stop_rule == 1 ? 1 : 0;
   if (p->iteration_limit == 0) {
     c.halt = 1;
     c.limit_hit = 1;
@@ -1302,22 +1455,22 @@ int pdhg_dev_time_kernels(pdhg_dev* h, int reps, double* primal_ms, double* dual
   const Iter<float> itf = iter_of<float>(h);
   HIP_TRY(h, hipEventRecord(e0, h->stream));
   for (int r = 0; r < reps && h->T.nitems > 0; ++r) {
-    if (mixed(h)) k_primal<float, float><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<float>(), itf, h->ctl, h->part_p, h->lr_p, 1);
-    else k_primal<double, double><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<double>(), itd, h->ctl, h->part_p, h->lr_p, 1);
+    if (mixed(h)) k_primal<float, float><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<float>(), itf, h->ctl, h->part_p, 1);
+    else k_primal<double, double><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<double>(), itd, h->ctl, h->part_p, 1);
   }
   HIP_TRY(h, hipEventRecord(e1, h->stream));
   for (int r = 0; r < reps && h->A.nitems > 0; ++r) {
-    if (mixed(h)) k_dual<float, float><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<float>(), itf, h->ctl, h->part_d, h->lr_d, 1);
-    else k_dual<double, double><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<double>(), itd, h->ctl, h->part_d, h->lr_d, 1);
+    if (mixed(h)) k_dual<float, float><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<float>(), itf, h->ctl, h->part_d, 1);
+    else k_dual<double, double><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<double>(), itd, h->ctl, h->part_d, 1);
   }
   HIP_TRY(h, hipEventRecord(e2, h->stream));
   HIP_TRY(h, hipEventSynchronize(e2));
   float a = 0.f, b = 0.f;
   HIP_TRY(h, hipEventElapsedTime(&a, e0, e1));
   HIP_TRY(h, hipEventElapsedTime(&b, e1, e2));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipEventDestroy(e2);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
   if (primal_ms) *primal_ms = a / reps;
   if (dual_ms) *dual_ms = b / reps;
   return PDHG_OK;

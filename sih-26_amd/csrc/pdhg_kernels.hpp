@@ -11,11 +11,13 @@
 //                  reduced with cross-lane shuffles. Short rows therefore never leave a lane
 //                  idle on a coalesced load.
 //   long item    - one segment of kSeg nonzeros of a row longer than kTile (the power-law
-//                  rows of a Zipf-distributed pattern). Each workgroup writes its segment sum;
-//                  the last of the row's workgroups to finish (an integer counter decides which,
-//                  not the order of the sum) adds the segment sums in index order and runs the
-//                  row's epilogue. No floating-point atomics are used anywhere, so results are
-//                  identical run to run.
+//                  rows of a Zipf-distributed pattern). The workgroup writes the segment's
+//                  sum and nothing else. A single "tail" workgroup launched afterwards adds each
+//                  row's segment sums in index order (staged through LDS) and runs that row's
+//                  epilogue. Workgroups never wait on or signal each other inside a kernel:
+//                  on gfx950 a device-scope release fence writes back the XCD's L2, and one per
+//                  workgroup costs more than the product itself. No floating-point atomics
+//                  are used anywhere, so results are identical run to run.
 //
 // The "epilogue" is the per-row work fused after the dot product: the PDHG primal or dual
 // update, or a convergence-test term. Each epilogue also feeds per-workgroup partial sums
@@ -30,7 +32,9 @@ namespace sankhya_hip {
 
 constexpr int kThreads = 256;   // 4 wave64 wavefronts
 constexpr int kTile = 2048;     // nonzeros per stream item
-constexpr int kSeg = 8192;      // nonzeros per long-row segment
+constexpr int kSeg = 4096;      // nonzeros per long-row segment
+constexpr int kTail = 1024;     // threads of the single-workgroup tail kernels
+constexpr int kChunk = 2048;    // long-row segment sums staged in LDS per tail round
 constexpr int kMaxRows = 256;   // rows per stream item
 
 __device__ __forceinline__ bool finite_bound(double v) {
@@ -46,6 +50,13 @@ __device__ __forceinline__ double project(double v, double lo, double hi) {
   if (finite_bound(lo) && v < lo) return lo;
   if (finite_bound(hi) && v > hi) return hi;
   return v;
+}
+
+// The matrix is streamed once per product: load it non-temporally so it does not evict the
+// gathered vectors from L2 and the MALL.
+template <typename T>
+__device__ __forceinline__ T stream_load(const T* p) {
+  return __builtin_nontemporal_load(p);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -73,7 +84,8 @@ struct Acc {
 };
 
 template <int NS, int NM>
-__device__ void block_reduce(Acc<NS, NM>& a, double* red /* >= 4*(NS+NM) doubles of LDS */) {
+__device__ void block_reduce(Acc<NS, NM>& a, double* red /* >= waves*(NS+NM) doubles of LDS */) {
+  const int nwaves = static_cast<int>(blockDim.x) >> 6;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NS; ++k) a.s[k] = wave_sum(a.s[k]);
@@ -91,13 +103,13 @@ __device__ void block_reduce(Acc<NS, NM>& a, double* red /* >= 4*(NS+NM) doubles
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
       double t = red[k];
-      for (int w = 1; w < kThreads / 64; ++w) t += red[w * (NS + NM) + k];
+      for (int w = 1; w < nwaves; ++w) t += red[w * (NS + NM) + k];
       a.s[k] = t;
     }
 #pragma unroll
     for (int k = 0; k < NM; ++k) {
       double t = red[NS + k];
-      for (int w = 1; w < kThreads / 64; ++w) t = smax(t, red[w * (NS + NM) + NS + k]);
+      for (int w = 1; w < nwaves; ++w) t = smax(t, red[w * (NS + NM) + NS + k]);
       a.m[k] = t;
     }
   }
@@ -108,8 +120,9 @@ __device__ void block_reduce(Acc<NS, NM>& a, double* red /* >= 4*(NS+NM) doubles
 // workgroup. Every thread returns the total.
 __device__ inline double block_sum_strided(const double* p, int64_t count, int stride, int idx,
                                            double* red, bool is_max) {
+  const int nwaves = static_cast<int>(blockDim.x) >> 6;
   double t = 0.0;
-  for (int64_t q = threadIdx.x; q < count; q += kThreads) {
+  for (int64_t q = threadIdx.x; q < count; q += blockDim.x) {
     const double v = p[q * stride + idx];
     t = is_max ? smax(t, v) : t + v;
   }
@@ -118,9 +131,43 @@ __device__ inline double block_sum_strided(const double* p, int64_t count, int s
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
   __syncthreads();
   double r = red[0];
-  for (int w = 1; w < kThreads / 64; ++w) r = is_max ? smax(r, red[w]) : r + red[w];
+  for (int w = 1; w < nwaves; ++w) r = is_max ? smax(r, red[w]) : r + red[w];
   __syncthreads();
   return r;
+}
+
+// NV sums (or maxima, where is_max[k]) over the concatenation of a[0..ca) and b[0..cb), each
+// entry NV wide at the given stride, in one pass: every lane walks its share in a fixed order,
+// then one cross-lane and one cross-wave step. Every thread returns the totals.
+template <int NV>
+__device__ inline void block_sums(const double* a, int64_t ca, const double* b, int64_t cb,
+                                  int stride, const bool (&is_max)[NV], double (&out)[NV],
+                                  double* red /* >= waves*NV */) {
+  const int nwaves = static_cast<int>(blockDim.x) >> 6;
+  double t[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) t[k] = 0.0;
+  const int64_t total = ca + cb;
+  for (int64_t q = threadIdx.x; q < total; q += blockDim.x) {
+    const double* row = q < ca ? a + q * stride : b + (q - ca) * stride;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) t[k] = is_max[k] ? smax(t[k], row[k]) : t[k] + row[k];
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) t[k] = is_max[k] ? wave_max(t[k]) : wave_sum(t[k]);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[(threadIdx.x >> 6) * NV + k] = t[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double r = red[k];
+    for (int w = 1; w < nwaves; ++w) r = is_max[k] ? smax(r, red[w * NV + k]) : r + red[w * NV + k];
+    out[k] = r;
+  }
+  __syncthreads();
 }
 
 // A CSR matrix with its work-item plan, as kernels see it.
@@ -132,10 +179,12 @@ struct PlanView {
   const int4* items;  // {r0, r1, k0, k1}; r1 < 0 marks a long-row segment of row r0 with id -r1-1
   const int32_t* lr_nseg;
   const int32_t* lr_base;
-  double* seg_part;   // segment sums, NR per segment
-  int32_t* lr_count;  // arrival counters, one per long row, zero between launches
+  const int32_t* lr_row;    // the row of each long row
+  const int2* lr_groups;    // [first, last) long rows whose segments fit one LDS round
+  double* seg_part;         // segment sums, NR per segment
   int32_t nitems;
   int32_t nlr;
+  int32_t ngroups;
 };
 
 template <int NR>
@@ -144,23 +193,35 @@ struct RowSum {
 };
 
 // Process this workgroup's item. gather(j) returns RowSum<NR> for column j; epi(row, sum,
-// acc) runs once per completed row (in one thread). Long-row epilogues accumulate into
-// `lr_acc`, which the caller stores per long row so its contribution does not depend on which
-// workgroup finished the row. Returns the long-row id whose epilogue ran here, or -1.
+// acc) runs once per completed row (in one thread). A long-row segment only stores its sum;
+// finish_long_rows completes those rows later.
 template <typename V, int NR, int NS, int NM, class G, class E>
-__device__ int run_item(const PlanView<V>& P, G& gather, E& epi, Acc<NS, NM>& acc,
-                        Acc<NS, NM>& lr_acc, double* lds, double* red) {
+__device__ void run_item(const PlanView<V>& P, G& gather, E& epi, Acc<NS, NM>& acc,
+                         double* lds, double* red) {
   const int4 it = P.items[blockIdx.x];
   const int tid = threadIdx.x;
   if (it.y >= 0) {
     const int r0 = it.x, r1 = it.y, k0 = it.z, k1 = it.w;
     const int cnt = k1 - k0;
-    for (int t = tid; t < cnt; t += kThreads) {
-      const int k = k0 + t;
-      const double a = static_cast<double>(P.val[k]);
-      const RowSum<NR> g = gather(P.col[k]);
+    // kTile / kThreads loads per thread, all issued before the first gather so each lane has
+    // that many independent index/value loads and then gathers in flight.
+    constexpr int U = kTile / kThreads;
+    int cidx[U];
+    double aval[U];
 #pragma unroll
-      for (int r = 0; r < NR; ++r) lds[r * kTile + t] = a * g.v[r];
+    for (int u = 0; u < U; ++u) {
+      const int t = u * kThreads + tid;
+      cidx[u] = t < cnt ? stream_load(P.col + k0 + t) : -1;
+      aval[u] = t < cnt ? static_cast<double>(stream_load(P.val + k0 + t)) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = u * kThreads + tid;
+      if (cidx[u] >= 0) {
+        const RowSum<NR> g = gather(cidx[u]);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) lds[r * kTile + t] = aval[u] * g.v[r];
+      }
     }
     __syncthreads();
     const int rows = r1 - r0;
@@ -183,44 +244,74 @@ __device__ int run_item(const PlanView<V>& P, G& gather, E& epi, Acc<NS, NM>& ac
       }
       if (lane == 0) epi(row, sum, acc);
     }
-    return -1;
+    return;
   }
-  // Long-row segment.
+  // Long-row segment: kSeg / kThreads independent loads per lane, then the gathers.
   const int row = it.x, lr = -it.y - 1, k0 = it.z, k1 = it.w;
   Acc<NR, 0> seg;
 #pragma unroll
   for (int r = 0; r < NR; ++r) seg.s[r] = 0.0;
-  for (int k = k0 + tid; k < k1; k += kThreads) {
-    const double a = static_cast<double>(P.val[k]);
-    const RowSum<NR> g = gather(P.col[k]);
+  {
+    constexpr int U = kSeg / kThreads;
+    int cidx[U];
+    double aval[U];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) seg.s[r] += a * g.v[r];
+    for (int u = 0; u < U; ++u) {
+      const int kk = k0 + u * kThreads + tid;
+      cidx[u] = kk < k1 ? stream_load(P.col + kk) : -1;
+      aval[u] = kk < k1 ? static_cast<double>(stream_load(P.val + kk)) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (cidx[u] >= 0) {
+        const RowSum<NR> g = gather(cidx[u]);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) seg.s[r] += aval[u] * g.v[r];
+      }
+    }
   }
   block_reduce(seg, red);
-  const int nseg = P.lr_nseg[lr];
-  const int base = P.lr_base[lr];
-  const int s = (k0 - P.rowptr[row]) / kSeg;
-  __shared__ int last;
   if (tid == 0) {
+    const int s = (k0 - P.rowptr[row]) / kSeg;
 #pragma unroll
-    for (int r = 0; r < NR; ++r) P.seg_part[(int64_t)(base + s) * NR + r] = seg.s[r];
-    __threadfence();
-    const int old = atomicAdd(&P.lr_count[lr], 1);
-    last = (old == nseg - 1) ? 1 : 0;
+    for (int r = 0; r < NR; ++r) P.seg_part[(int64_t)(P.lr_base[lr] + s) * NR + r] = seg.s[r];
   }
-  __syncthreads();
-  if (!last) return -1;
-  __threadfence();
-  RowSum<NR> total;
+}
+
+// Completes every long row of P in one workgroup (launched after the kernel that wrote the
+// segment sums): for each group of long rows, stage their segment sums in LDS with coalesced
+// loads, then one thread per row adds its segments in index order and runs epi.
+template <typename V, int NR, int NS, int NM, class E>
+__device__ void finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
+                                 double* lds /* kChunk * NR */) {
+  for (int g = 0; g < P.ngroups; ++g) {
+    const int2 grp = P.lr_groups[g];
+    const int base0 = P.lr_base[grp.x];
+    const int cnt = P.lr_base[grp.y - 1] + P.lr_nseg[grp.y - 1] - base0;
+    for (int t = threadIdx.x; t < cnt * NR; t += blockDim.x) lds[t] = P.seg_part[(int64_t)base0 * NR + t];
+    __syncthreads();
+    // L lanes per row (a power of two up to a wavefront), reduced with cross-lane shuffles.
+    const int nrows = grp.y - grp.x;
+    int L = 64;
+    while (L > 1 && nrows * L > static_cast<int>(blockDim.x)) L >>= 1;
+    const int gi = threadIdx.x / L, lane = threadIdx.x & (L - 1), ng = blockDim.x / L;
+    for (int lr = grp.x + gi; lr < grp.y; lr += ng) {
+      const int s0 = P.lr_base[lr] - base0, ns = P.lr_nseg[lr];
+      RowSum<NR> sum;
 #pragma unroll
-  for (int r = 0; r < NR; ++r)
-    total.v[r] = block_sum_strided(P.seg_part + (int64_t)base * NR, nseg, NR, r, red, false);
-  if (tid == 0) {
-    lr_acc.zero();
-    epi(row, total, lr_acc);
-    atomicExch(&P.lr_count[lr], 0);
+      for (int r = 0; r < NR; ++r) sum.v[r] = 0.0;
+      for (int s = lane; s < ns; s += L) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) sum.v[r] += lds[(s0 + s) * NR + r];
+      }
+      for (int off = L >> 1; off > 0; off >>= 1) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) sum.v[r] += __shfl_xor(sum.v[r], off);
+      }
+      if (lane == 0) epi(P.lr_row[lr], sum, acc);
+    }
+    __syncthreads();
   }
-  return lr;
 }
 
 }  // namespace sankhya_hip

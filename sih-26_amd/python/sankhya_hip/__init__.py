@@ -59,7 +59,7 @@ class DevParams(C.Structure):
     _fields_ = [
         ("tolerance", C.c_double), ("iteration_limit", C.c_int64), ("restart", C.c_int32),
         ("stop_at_request", C.c_int32), ("spectral_norm", C.c_double),
-        ("chunk_attempts", C.c_int32),
+        ("chunk_attempts", C.c_int32), ("stop_rule", C.c_int32),
     ]
 
 
@@ -310,9 +310,9 @@ class Device:
         return act[: self.m], red[: self.n], res
 
     def start(self, tolerance=1e-4, iteration_limit=1000000, restart=True, stop_at_request=False,
-              spectral_norm=1.0, chunk=0):
+              spectral_norm=1.0, chunk=0, stop_rule=0):
         p = DevParams(tolerance, iteration_limit, 1 if restart else 0, 1 if stop_at_request else 0,
-                      spectral_norm, chunk)
+                      spectral_norm, chunk, stop_rule)
         self._check(self._lib.pdhg_dev_start(self._h, C.byref(p)), "start")
 
     def run(self, chunks: int = 1) -> DevState:

@@ -53,6 +53,12 @@ std::vector<OptionSpec> build_registry() {
   s.push_back({"pdhg_precision", OptionType::String, std::string("fp64"),
                "Bulk arithmetic: fp64, or mixed (FP32 matrix and iterates, FP64 reductions, step "
                "control and residuals).", 0, 0, {"fp64", "mixed"}});
+  s.push_back({"pdhg_stop_rule", OptionType::String, std::string("reference"),
+               "reference: stop as SANKHYA's PDHG does (relative residuals <= pdhg_tolerance and "
+               "absolute primal <= 1e-7, and the project standard unless pdhg_stop_at_request). "
+               "relative_kkt: stop when the relative KKT error, max(relative primal, dual, gap), "
+               "is <= pdhg_tolerance; the report still only claims optimal on the standard.",
+               0, 0, {"reference", "relative_kkt"}});
   s.push_back({"pdhg_chunk", OptionType::Int, std::int64_t{0},
                "Attempts per captured graph chunk; 0 picks the default.", 0.0, 4096.0});
   s.push_back({"log_level", OptionType::String, std::string("info"), "off, error, warning, info, verbose, debug.",

@@ -200,6 +200,7 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   params.stop_at_request = stop_at_request ? 1 : 0;
   params.spectral_norm = spectral_norm;
   params.chunk_attempts = static_cast<int32_t>(options.get_int("pdhg_chunk"));
+  params.stop_rule = options.get_string("pdhg_stop_rule") == "relative_kkt" ? 1 : 0;
   rc = pdhg_dev_start(dev.h, &params);
   if (rc != PDHG_OK) return dev_error(rc, "start");
   const double loop_start = timer.elapsed_seconds();

@@ -1,0 +1,152 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports what include/*.h
+declares, the engine's host preconditioner equals the oracle's (and so the reference's) bit
+for bit, the generators are deterministic and have the documented shapes, options and model
+errors are refused, and a missing device is an error, never a silent CPU fallback."""
+import math
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import common as cm
+import oracle as orc
+import sankhya_hip as sh
+
+ROOT = os.path.dirname(cm.HERE)
+
+
+def declared_functions(header):
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = set()
+    for m in re.finditer(r"\b([a-z_][a-z0-9_]*)\s*\(", text):
+        name = m.group(1)
+        before = text[max(0, m.start() - 40): m.start()]
+        if name in ("if", "for", "while", "return", "sizeof", "defined"):
+            continue
+        if re.search(r"(int|void|double|int64_t|char\s*\*|pdhg_dev\s*\*|sankhya_model\s*\*|uint32_t)\s*\**\s*$", before):
+            names.add(name)
+    return names
+
+
+@pytest.mark.parametrize("header", ["pdhg_hip.h", "sankhya_pdhg.h"])
+def test_library_exports_every_declared_function(header):
+    lib = sh.load_library()
+    names = declared_functions(os.path.join(ROOT, "include", header))
+    assert len(names) >= 8
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_oracle_exports_every_declared_function():
+    names = declared_functions(os.path.join(ROOT, "oracle", "pdhg_oracle.h"))
+    missing = [n for n in sorted(names) if not hasattr(orc.lib(), n)]
+    assert not missing and len(names) >= 8, missing
+
+
+def _instances():
+    yield "C0-s1", cm.c0(1)
+    yield "C0-s3", cm.c0(3)
+    yield "zipf-2000x5000", cm.generated("zipf", 2000, 5000, 10, 3)
+    yield "blend-500x1200", cm.generated("blend", 500, 1200, 4, 5)
+    for nm in ("afiro", "adlittle", "israel", "share2b"):
+        d = cm.netlib(nm)
+        yield nm, (cm.model_of(d), d)
+
+
+@pytest.mark.parametrize("name,inst", list(_instances()), ids=[c[0] for c in _instances()])
+def test_host_scaling_matches_oracle_bit_for_bit(name, inst):
+    mdl, d = inst
+    a = sh.host_scaling(mdl)
+    b = orc.build_scaling(orc.Problem(d))
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_generators_are_deterministic_and_shaped():
+    m1, d1 = cm.c0(4)
+    m2, d2 = cm.c0(4)
+    for k in ("colptr", "rowidx", "val", "cost", "row_lower"):
+        assert np.array_equal(d1[k], d2[k])
+    _, d3 = cm.c0(5)
+    assert not np.array_equal(d1["val"], d3["val"])
+    # C0: 5 distinct rows per column, all equality rows, box [0, 10]
+    assert d1["nnz"] == 5 * 3000 and np.all(np.diff(d1["colptr"]) == 5)
+    for j in range(0, 3000, 97):
+        r = d1["rowidx"][d1["colptr"][j]: d1["colptr"][j + 1]]
+        assert np.all(np.diff(r) > 0)
+    assert np.array_equal(d1["row_lower"], d1["row_upper"])
+    assert np.all(d1["col_lower"] == 0) and np.all(d1["col_upper"] == 10)
+    # zipf: 30% <= rows, power-law row lengths with a near-dense first row
+    _, z = cm.generated("zipf", 20000, 50000, 10, 2)
+    assert np.isinf(z["row_lower"]).sum() == 6000
+    cnt = np.bincount(z["rowidx"], minlength=20000)
+    assert cnt[0] > 0.9 * 50000 and cnt[0] > cnt[1] > cnt[10] > cnt[1000]
+    # blend: a dense last row sum x = const, 40% of columns upper-bounded
+    _, b = cm.generated("blend", 500, 1200, 4, 5)
+    assert np.count_nonzero(b["rowidx"] == 499) == 1200
+    assert np.isfinite(b["col_upper"]).sum() == 480
+    assert np.all((b["cost"] >= -5) & (b["cost"] <= 5))
+
+
+def test_generated_instances_are_feasible_at_the_planted_point():
+    # b = A x0 with x0 in [0, 1]: the rows hold at x0 up to rounding. x0 is not exported, so
+    # check the equivalent: the LP relaxation solved by the oracle reaches optimal.
+    mdl = sh.Model.generate("uniform", 60, 180, 5, 2)
+    r = orc.solve(orc.Problem(mdl.arrays()), tolerance=1e-8, iteration_limit=400000)
+    assert r["status"] == "optimal"
+
+
+def test_options_are_validated():
+    mdl, _ = cm.c0(1)
+    with pytest.raises(ValueError, match="unknown option"):
+        sh.solve(mdl, no_such_option=1)
+    with pytest.raises(ValueError, match="out of range"):
+        sh.solve(mdl, pdhg_tolerance=2.0)
+    with pytest.raises(ValueError, match="permitted"):
+        sh.solve(mdl, pdhg_precision="fp16")
+
+
+def test_model_errors_are_reported_before_any_device_work():
+    d = cm.dense_lp([[1, 1]], [2], [math.inf], [1, 1])
+    d["col_lower"] = np.array([0.0, np.nan])
+    s = sh.solve(cm.model_of(d))
+    assert s.status == "model_error" and "NaN" in s.message
+    d = cm.dense_lp([[1, 1]], [5], [2], [1, 1])
+    s = sh.solve(cm.model_of(d))
+    assert s.status == "model_error" and "above upper bound" in s.message
+
+
+def test_no_device_is_an_error_not_a_cpu_fallback():
+    ok, _ = sh.device_description(0)
+    if ok:
+        pytest.skip("a gfx950 device is present")
+    mdl, _ = cm.c0(1)
+    s = sh.solve(mdl, pdhg_tolerance=1e-6)
+    assert s.status == "numerical_error"
+    assert "gfx950" in s.message and s.algorithm == "pdhg-gpu"
+    assert s.iterations == 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 2 and "one GPU" in p.stdout
+
+
+def test_verifier_restatement_rejects_planted_errors():
+    # The acceptance check itself: a correct optimum passes, a perturbed one fails.
+    from verify import accepted
+    d = cm.dense_lp([[1, 1]], [4], [4], [1, 2])
+    good = dict(status="optimal", objective=4.0, col_value=[4.0, 0.0], row_dual=[1.0],
+                col_dual=[0.0, 1.0], row_activity=[4.0])
+    ok, checks = accepted(d, good)
+    assert ok, checks
+    bad = dict(good, col_value=[3.9, 0.0], objective=3.9, row_activity=[3.9])
+    assert not accepted(d, bad)[0]
+    bad = dict(good, row_dual=[2.0])
+    assert not accepted(d, bad)[0]
