@@ -416,19 +416,29 @@ __global__ __launch_bounds__(kTail) void k_dual_tail(PlanView<V> A, Iter<X> it, 
                                                      const double* lr_p, int has_lr_p,
                                                      const double* part_d, int nd, int force) {
   __shared__ double lds[2 * kChunk];
-  __shared__ double red[16 * 2];
+  __shared__ double red[16 * 5];
   if (!force && ctl->halt) return;
   DualEpi<X> epi = dual_epi(it, ctl);
+  // s[0] primal movement, s[1] dual movement, s[2] interaction (per-workgroup partials);
+  // s[3], s[4] the same two dual terms from the long rows.
+  Acc<5, 0> a;
+  a.zero();
+  auto partials = [&]() {
+    double p[1] = {0.0}, d[2] = {0.0, 0.0};
+    lane_sums<1>(part_p, np, 1, p);
+    lane_sums<2>(part_d, nd, 2, d);
+    if (has_lr_p && threadIdx.x == 0) p[0] += lr_p[0];
+    a.s[0] = p[0];
+    a.s[1] = d[0];
+    a.s[2] = d[1];
+  };
   Acc<2, 0> lacc;
   lacc.zero();
-  finish_long_rows<V, 2>(A, epi, lacc, lds);
-  block_reduce(lacc, red);
-  constexpr bool kSum1[1] = {false};
-  constexpr bool kSum2[2] = {false, false};
-  double p[1], d[2];
-  block_sums<1>(part_p, np, lr_p, has_lr_p ? 1 : 0, 1, kSum1, p, red);
-  block_sums<2>(part_d, nd, nullptr, 0, 2, kSum2, d, red);
-  if (threadIdx.x == 0) step_decide(ctl, p[0], d[0] + lacc.s[0], d[1] + lacc.s[1]);
+  finish_long_rows<V, 2>(A, epi, lacc, lds, partials);
+  a.s[3] = lacc.s[0];
+  a.s[4] = lacc.s[1];
+  block_reduce(a, red);
+  if (threadIdx.x == 0) step_decide(ctl, a.s[0], a.s[1] + a.s[3], a.s[2] + a.s[4]);
 }
 
 template <typename V, typename X>
@@ -838,24 +848,46 @@ struct HostPlan {
   int32_t nseg = 0;
 };
 
-// Work items over a CSR row pointer: whole rows packed up to kTile nonzeros and kMaxRows
-// rows per item; rows longer than kTile split into kSeg segments.
-HostPlan build_plan(int32_t rows, const int32_t* rowptr) {
+// Work items over a CSR matrix: whole rows packed up to kTile nonzeros and kMaxRows rows per
+// item; rows longer than kTile split into segments of at most kSeg nonzeros that never cross a
+// column band (ncols split into kXcds equal bands).
+//
+// XCD-aware dispatch: workgroup b runs on XCD b % kXcds, and each XCD has its own L2. The
+// segments of band k are placed at item positions congruent to k, so every XCD's L2 only has to
+// hold its band of the gathered vector rather than all of it. The `logical` order (rows
+// ascending, segments of a row in column order) is kept for the coverage check.
+constexpr int kXcds = 8;
+
+HostPlan build_plan(int32_t rows, int32_t ncols, const int32_t* rowptr, const int32_t* col,
+                    std::vector<int4>* logical) {
   HostPlan hp;
+  std::vector<int4> stream;
+  std::vector<std::vector<int4>> band(kXcds);
+  const int64_t width = std::max<int64_t>(1, (static_cast<int64_t>(ncols) + kXcds - 1) / kXcds);
   int32_t r = 0;
   while (r < rows) {
     const int32_t len = rowptr[r + 1] - rowptr[r];
     if (len > kTile) {
       const int32_t lr = static_cast<int32_t>(hp.lr_nseg.size());
-      const int32_t nseg = (len + kSeg - 1) / kSeg;
+      int32_t nseg = 0;
+      int32_t k = rowptr[r];
+      while (k < rowptr[r + 1]) {
+        const int b = static_cast<int>(std::min<int64_t>(col[k] / width, kXcds - 1));
+        const int64_t band_end_col = (b + 1) * width;
+        int32_t kb = k;  // first nonzero of the next band
+        while (kb < rowptr[r + 1] && col[kb] < band_end_col) ++kb;
+        for (int32_t k0 = k; k0 < kb; k0 += kSeg) {
+          const int32_t k1 = std::min(k0 + kSeg, kb);
+          const int4 item = make_int4(hp.nseg + nseg, -lr - 1, k0, k1);
+          band[b].push_back(item);
+          logical->push_back(item);
+          ++nseg;
+        }
+        k = kb;
+      }
       hp.lr_nseg.push_back(nseg);
       hp.lr_base.push_back(hp.nseg);
       hp.lr_row.push_back(r);
-      for (int32_t s = 0; s < nseg; ++s) {
-        const int32_t k0 = rowptr[r] + s * kSeg;
-        const int32_t k1 = std::min(rowptr[r] + (s + 1) * kSeg, rowptr[r + 1]);
-        hp.items.push_back(make_int4(r, -lr - 1, k0, k1));
-      }
       hp.nseg += nseg;
       ++r;
       continue;
@@ -868,7 +900,27 @@ HostPlan build_plan(int32_t rows, const int32_t* rowptr) {
       acc += l;
       ++r;
     }
-    hp.items.push_back(make_int4(r0, r, rowptr[r0], rowptr[r]));
+    const int4 item = make_int4(r0, r, rowptr[r0], rowptr[r]);
+    stream.push_back(item);
+    logical->push_back(item);
+  }
+  // Dispatch order: position p runs on XCD p % kXcds; take that band's next segment, else a
+  // stream item, else the fullest band's.
+  std::vector<size_t> next(kXcds, 0);
+  size_t ns = 0;
+  const size_t total = logical->size();
+  for (size_t p = 0; p < total; ++p) {
+    const int x = static_cast<int>(p % kXcds);
+    if (next[x] < band[x].size()) {
+      hp.items.push_back(band[x][next[x]++]);
+    } else if (ns < stream.size()) {
+      hp.items.push_back(stream[ns++]);
+    } else {
+      int best = 0;
+      for (int b = 1; b < kXcds; ++b)
+        if (band[b].size() - next[b] > band[best].size() - next[best]) best = b;
+      hp.items.push_back(band[best][next[best]++]);
+    }
   }
   const int32_t nlr = static_cast<int32_t>(hp.lr_nseg.size());
   for (int32_t lr = 0; lr < nlr;) {
@@ -958,52 +1010,75 @@ int upload(pdhg_dev* h, T** dst, const T* src, size_t count) {
   return PDHG_OK;
 }
 
-// Every row covered exactly once, stream items within the LDS tile and the row cap, long
-// segments within kSeg: checked on the host before any kernel sees the plan.
-bool plan_ok(const HostPlan& hp, int32_t rows, const int32_t* rowptr) {
-  int32_t next_row = 0;
-  for (const int4& it : hp.items) {
+// Every row covered exactly once (in the logical order), stream items within the LDS tile
+// and the row cap, long segments within kSeg and numbered consecutively per row, and the
+// dispatch order a permutation of the logical one: checked on the host before any kernel sees
+// the plan.
+bool plan_ok(const HostPlan& hp, const std::vector<int4>& logical, int32_t rows,
+             const int32_t* rowptr) {
+  int32_t next_row = 0, next_k = -1;
+  for (const int4& it : logical) {
     if (it.y >= 0) {
+      if (next_k >= 0) return false;
       if (it.x != next_row || it.y <= it.x || it.y - it.x > kMaxRows) return false;
       if (it.z != rowptr[it.x] || it.w != rowptr[it.y] || it.w - it.z > kTile) return false;
       next_row = it.y;
     } else {
       const int32_t lr = -it.y - 1;
       if (lr < 0 || lr >= static_cast<int32_t>(hp.lr_nseg.size())) return false;
-      if (it.z < rowptr[it.x] || it.w > rowptr[it.x + 1] || it.w <= it.z || it.w - it.z > kSeg) return false;
-      if (it.w == rowptr[it.x + 1]) {
-        if (it.x != next_row) return false;
-        next_row = it.x + 1;
+      const int32_t row = hp.lr_row[lr];
+      if (row != next_row) return false;
+      if (it.x < hp.lr_base[lr] || it.x >= hp.lr_base[lr] + hp.lr_nseg[lr]) return false;
+      if (it.z != (next_k < 0 ? rowptr[row] : next_k) || it.w <= it.z || it.w - it.z > kSeg) return false;
+      next_k = it.w;
+      if (it.w == rowptr[row + 1]) {
+        next_row = row + 1;
+        next_k = -1;
       }
     }
   }
+  if (hp.items.size() != logical.size()) return false;
   // A single long row must fit one LDS round of the tail kernels (kChunk segment sums).
   for (const int32_t ns : hp.lr_nseg)
     if (ns > kChunk) return false;
-  return next_row == rows;
+  return next_row == rows && next_k < 0;
 }
 
-int make_plan(pdhg_dev* h, DevPlan* P, int32_t rows, int64_t nnz, const int32_t* rowptr,
-              const int32_t* col, const double* val) {
-  const HostPlan hp = build_plan(rows, rowptr);
-  if (!plan_ok(hp, rows, rowptr)) return fail(h, PDHG_E_ARGUMENT, "internal: work plan does not cover the matrix");
+int make_plan(pdhg_dev* h, DevPlan* P, int32_t rows, int32_t ncols, int64_t nnz,
+              const int32_t* rowptr, const int32_t* col, const double* val) {
+  std::vector<int4> logical;
+  const HostPlan hp = build_plan(rows, ncols, rowptr, col, &logical);
+  if (!plan_ok(hp, logical, rows, rowptr))
+    return fail(h, PDHG_E_ARGUMENT, "internal: work plan does not cover the matrix");
   P->rows = rows;
   P->nitems = static_cast<int32_t>(hp.items.size());
   P->nlr = static_cast<int32_t>(hp.lr_nseg.size());
   P->nseg = hp.nseg;
   int rc;
   if ((rc = upload(h, &P->rowptr, rowptr, static_cast<size_t>(rows) + 1))) return rc;
-  if ((rc = upload(h, &P->col, col, static_cast<size_t>(nnz)))) return rc;
+  // Index and value arrays carry kPad zeroed entries past nnz: the kernels load nonzeros in
+  // aligned pairs, so the last pair of an odd-length array reads one entry past the end.
+  constexpr size_t kPad = 2;
+  const size_t un = static_cast<size_t>(nnz);
+  {
+    std::vector<int32_t> c(un + kPad, 0);
+    std::copy(col, col + nnz, c.begin());
+    if ((rc = upload(h, &P->col, c.data(), c.size()))) return rc;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+  }
   if (h->precision == PDHG_PRECISION_MIXED) {
-    std::vector<float> f(static_cast<size_t>(nnz));
+    std::vector<float> f(un + kPad, 0.0f);
     for (int64_t k = 0; k < nnz; ++k) f[k] = static_cast<float>(val[k]);
     float* d = nullptr;
     if ((rc = upload(h, &d, f.data(), f.size()))) return rc;
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     P->val = d;
   } else {
+    std::vector<double> v(un + kPad, 0.0);
+    std::copy(val, val + nnz, v.begin());
     double* d = nullptr;
-    if ((rc = upload(h, &d, val, static_cast<size_t>(nnz)))) return rc;
+    if ((rc = upload(h, &d, v.data(), v.size()))) return rc;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
     P->val = d;
   }
   if ((rc = upload(h, &P->items, hp.items.data(), hp.items.size()))) return rc;
@@ -1206,8 +1281,8 @@ int pdhg_dev_upload(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz, const int32_
   h->n = n;
   h->nnz = nnz;
   int rc;
-  if ((rc = make_plan(h, &h->A, m, nnz, a_rowptr, a_colidx, a_val))) return rc;
-  if ((rc = make_plan(h, &h->T, n, nnz, at_rowptr, at_colidx, at_val))) return rc;
+  if ((rc = make_plan(h, &h->A, m, n, nnz, a_rowptr, a_colidx, a_val))) return rc;
+  if ((rc = make_plan(h, &h->T, n, m, nnz, at_rowptr, at_colidx, at_val))) return rc;
   const size_t xs = mixed(h) ? sizeof(float) : sizeof(double);
   {
     char* p = nullptr;

@@ -30,12 +30,19 @@
 
 namespace sankhya_hip {
 
-constexpr int kThreads = 256;   // 4 wave64 wavefronts
-constexpr int kTile = 2048;     // nonzeros per stream item
-constexpr int kSeg = 4096;      // nonzeros per long-row segment
+// Work-item geometry; the defaults are the tuned values, the macros exist for sweeps.
+#ifndef SANKHYA_TILE
+#define SANKHYA_TILE 2048
+#endif
+#ifndef SANKHYA_SEG
+#define SANKHYA_SEG 4096
+#endif
+constexpr int kThreads = 256;           // 4 wave64 wavefronts
+constexpr int kTile = SANKHYA_TILE;     // nonzeros per stream item
+constexpr int kSeg = SANKHYA_SEG;       // nonzeros per long-row segment
 constexpr int kTail = 1024;     // threads of the single-workgroup tail kernels
 constexpr int kChunk = 2048;    // long-row segment sums staged in LDS per tail round
-constexpr int kMaxRows = 256;   // rows per stream item
+constexpr int kMaxRows = 1024;  // rows per stream item
 
 __device__ __forceinline__ bool finite_bound(double v) {
   // include/sankhya/types.hpp:89-96: absent when infinite or at/after the MPS 1e30 marker.
@@ -57,6 +64,22 @@ __device__ __forceinline__ double project(double v, double lo, double hi) {
 template <typename T>
 __device__ __forceinline__ T stream_load(const T* p) {
   return __builtin_nontemporal_load(p);
+}
+__device__ __forceinline__ int2 stream_load(const int2* p) {
+  const long long v = __builtin_nontemporal_load(reinterpret_cast<const long long*>(p));
+  return make_int2(static_cast<int>(v & 0xffffffffLL), static_cast<int>(v >> 32));
+}
+// Two consecutive matrix values (the pointer is pair-aligned), widened to double.
+__device__ __forceinline__ double2 load_vals2(const double* p) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2 v = __builtin_nontemporal_load(reinterpret_cast<const d2*>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ double2 load_vals2(const float* p) {
+  const double v = __builtin_nontemporal_load(reinterpret_cast<const double*>(p));
+  float2 f;
+  __builtin_memcpy(&f, &v, sizeof(f));
+  return make_double2(static_cast<double>(f.x), static_cast<double>(f.y));
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -148,10 +171,21 @@ __device__ inline void block_sums(const double* a, int64_t ca, const double* b, 
 #pragma unroll
   for (int k = 0; k < NV; ++k) t[k] = 0.0;
   const int64_t total = ca + cb;
-  for (int64_t q = threadIdx.x; q < total; q += blockDim.x) {
-    const double* row = q < ca ? a + q * stride : b + (q - ca) * stride;
+  constexpr int U = 4;  // entries per lane whose loads are issued before any is used
+  for (int64_t q0 = threadIdx.x; q0 < total; q0 += static_cast<int64_t>(blockDim.x) * U) {
+    double v[U][NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) t[k] = is_max[k] ? smax(t[k], row[k]) : t[k] + row[k];
+    for (int u = 0; u < U; ++u) {
+      const int64_t q = q0 + static_cast<int64_t>(u) * blockDim.x;
+      const double* row = q < ca ? a + q * stride : b + (q - ca) * stride;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) v[u][k] = q < total ? row[k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) t[k] = is_max[k] ? smax(t[k], v[u][k]) : t[k] + v[u][k];
+    }
   }
 #pragma unroll
   for (int k = 0; k < NV; ++k) t[k] = is_max[k] ? wave_max(t[k]) : wave_sum(t[k]);
@@ -176,7 +210,7 @@ struct PlanView {
   const int32_t* rowptr;
   const int32_t* col;
   const V* val;
-  const int4* items;  // {r0, r1, k0, k1}; r1 < 0 marks a long-row segment of row r0 with id -r1-1
+  const int4* items;  // {r0, r1, k0, k1}; r1 < 0: a long-row segment {seg id, -lr-1, k0, k1}
   const int32_t* lr_nseg;
   const int32_t* lr_base;
   const int32_t* lr_row;    // the row of each long row
@@ -203,24 +237,34 @@ __device__ void run_item(const PlanView<V>& P, G& gather, E& epi, Acc<NS, NM>& a
   if (it.y >= 0) {
     const int r0 = it.x, r1 = it.y, k0 = it.z, k1 = it.w;
     const int cnt = k1 - k0;
-    // kTile / kThreads loads per thread, all issued before the first gather so each lane has
-    // that many independent index/value loads and then gathers in flight.
-    constexpr int U = kTile / kThreads;
-    int cidx[U];
-    double aval[U];
+    // Two nonzeros per lane per step (one 16-byte value load, one 8-byte index load), from a
+    // pair-aligned base; kTile / (2 kThreads) steps, all loads issued before the first gather.
+    const int kb = k0 & ~1;
+    constexpr int U = kTile / (2 * kThreads) + 1;
+    int2 cidx[U];
+    double2 aval[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int t = u * kThreads + tid;
-      cidx[u] = t < cnt ? stream_load(P.col + k0 + t) : -1;
-      aval[u] = t < cnt ? static_cast<double>(stream_load(P.val + k0 + t)) : 0.0;
+      const int k = kb + 2 * (u * kThreads + tid);
+      if (k < k1) {
+        cidx[u] = stream_load(reinterpret_cast<const int2*>(P.col + k));
+        aval[u] = load_vals2(P.val + k);
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int t = u * kThreads + tid;
-      if (cidx[u] >= 0) {
-        const RowSum<NR> g = gather(cidx[u]);
+      const int k = kb + 2 * (u * kThreads + tid);
+      if (k < k1) {
+        if (k >= k0) {
+          const RowSum<NR> g = gather(cidx[u].x);
 #pragma unroll
-        for (int r = 0; r < NR; ++r) lds[r * kTile + t] = aval[u] * g.v[r];
+          for (int r = 0; r < NR; ++r) lds[r * kTile + (k - k0)] = aval[u].x * g.v[r];
+        }
+        if (k + 1 < k1) {
+          const RowSum<NR> g = gather(cidx[u].y);
+#pragma unroll
+          for (int r = 0; r < NR; ++r) lds[r * kTile + (k + 1 - k0)] = aval[u].y * g.v[r];
+        }
       }
     }
     __syncthreads();
@@ -246,49 +290,83 @@ __device__ void run_item(const PlanView<V>& P, G& gather, E& epi, Acc<NS, NM>& a
     }
     return;
   }
-  // Long-row segment: kSeg / kThreads independent loads per lane, then the gathers.
-  const int row = it.x, lr = -it.y - 1, k0 = it.z, k1 = it.w;
+  // Long-row segment {global segment id, -(long row id)-1, k0, k1}: kSeg / kThreads
+  // independent loads per lane, then the gathers.
+  const int seg_id = it.x, k0 = it.z, k1 = it.w;
   Acc<NR, 0> seg;
 #pragma unroll
   for (int r = 0; r < NR; ++r) seg.s[r] = 0.0;
   {
-    constexpr int U = kSeg / kThreads;
-    int cidx[U];
-    double aval[U];
+    const int kb = k0 & ~1;
+    constexpr int U = kSeg / (2 * kThreads) + 1;
+    int2 cidx[U];
+    double2 aval[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int kk = k0 + u * kThreads + tid;
-      cidx[u] = kk < k1 ? stream_load(P.col + kk) : -1;
-      aval[u] = kk < k1 ? static_cast<double>(stream_load(P.val + kk)) : 0.0;
+      const int k = kb + 2 * (u * kThreads + tid);
+      if (k < k1) {
+        cidx[u] = stream_load(reinterpret_cast<const int2*>(P.col + k));
+        aval[u] = load_vals2(P.val + k);
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (cidx[u] >= 0) {
-        const RowSum<NR> g = gather(cidx[u]);
+      const int k = kb + 2 * (u * kThreads + tid);
+      if (k < k1) {
+        if (k >= k0) {
+          const RowSum<NR> g = gather(cidx[u].x);
 #pragma unroll
-        for (int r = 0; r < NR; ++r) seg.s[r] += aval[u] * g.v[r];
+          for (int r = 0; r < NR; ++r) seg.s[r] += aval[u].x * g.v[r];
+        }
+        if (k + 1 < k1) {
+          const RowSum<NR> g = gather(cidx[u].y);
+#pragma unroll
+          for (int r = 0; r < NR; ++r) seg.s[r] += aval[u].y * g.v[r];
+        }
       }
     }
   }
   block_reduce(seg, red);
   if (tid == 0) {
-    const int s = (k0 - P.rowptr[row]) / kSeg;
 #pragma unroll
-    for (int r = 0; r < NR; ++r) P.seg_part[(int64_t)(P.lr_base[lr] + s) * NR + r] = seg.s[r];
+    for (int r = 0; r < NR; ++r) P.seg_part[(int64_t)seg_id * NR + r] = seg.s[r];
   }
 }
 
 // Completes every long row of P in one workgroup (launched after the kernel that wrote the
 // segment sums): for each group of long rows, stage their segment sums in LDS with coalesced
-// loads, then one thread per row adds its segments in index order and runs epi.
-template <typename V, int NR, int NS, int NM, class E>
+// loads, then a power-of-two group of lanes per row adds its segments in index order and one
+// lane runs epi. `pre()` runs once, between issuing the first group's loads and using them, so
+// a caller's own independent loads share that memory round trip.
+struct NoPre {
+  __device__ void operator()() const {}
+};
+
+template <typename V, int NR, int NS, int NM, class E, class Pre = NoPre>
 __device__ void finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
-                                 double* lds /* kChunk * NR */) {
+                                 double* lds /* kChunk * NR */, Pre pre = Pre()) {
+  bool pre_done = false;
   for (int g = 0; g < P.ngroups; ++g) {
     const int2 grp = P.lr_groups[g];
     const int base0 = P.lr_base[grp.x];
     const int cnt = P.lr_base[grp.y - 1] + P.lr_nseg[grp.y - 1] - base0;
-    for (int t = threadIdx.x; t < cnt * NR; t += blockDim.x) lds[t] = P.seg_part[(int64_t)base0 * NR + t];
+    // kChunk * NR / kTail loads per lane at most, held in registers until `pre` has run.
+    constexpr int U = (kChunk * NR + kTail - 1) / kTail;
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = u * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+      v[u] = t < cnt * NR ? P.seg_part[(int64_t)base0 * NR + t] : 0.0;
+    }
+    if (!pre_done) {
+      pre();
+      pre_done = true;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = u * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+      if (t < cnt * NR) lds[t] = v[u];
+    }
     __syncthreads();
     // L lanes per row (a power of two up to a wavefront), reduced with cross-lane shuffles.
     const int nrows = grp.y - grp.x;
@@ -311,6 +389,29 @@ __device__ void finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
       if (lane == 0) epi(P.lr_row[lr], sum, acc);
     }
     __syncthreads();
+  }
+  if (!pre_done) pre();
+}
+
+// Adds p[q * stride + k] for q in [0, count) into out[k], k < NV, with the loads of several
+// entries per lane issued before they are used. Every lane keeps its own partial sums.
+template <int NV>
+__device__ __forceinline__ void lane_sums(const double* p, int64_t count, int stride,
+                                          double (&out)[NV]) {
+  constexpr int U = 4;
+  for (int64_t q0 = threadIdx.x; q0 < count; q0 += static_cast<int64_t>(blockDim.x) * U) {
+    double v[U][NV];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t q = q0 + static_cast<int64_t>(u) * blockDim.x;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) v[u][k] = q < count ? p[q * stride + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) out[k] += v[u][k];
+    }
   }
 }
 
