@@ -120,6 +120,15 @@ int pdhg_dev_start(pdhg_dev* h, const pdhg_params* p);
  * the state. Ticks produced are appended to an internal log read by pdhg_dev_ticks. */
 int pdhg_dev_run(pdhg_dev* h, int chunks, pdhg_state* state);
 
+/* Mixed-precision handoff (no reference counterpart: the reference is FP64 throughout). On a
+ * PDHG_PRECISION_MIXED handle still in its FP32 phase: widens the iterate pairs to FP64
+ * (exactly), keeps every other piece of state (step size, primal weight, running averages,
+ * restart point, best point, counters), installs the new tolerance, stopping rule and
+ * iteration limit from p, clears the stop flag, and recaptures the graph chunk with FP64
+ * matrix values. The next pdhg_dev_run continues the same run in FP64 (the "FP64 polish").
+ * The evaluation ticks and pdhg_dev_finish read FP64 values in both phases. */
+int pdhg_dev_promote(pdhg_dev* h, const pdhg_params* p);
+
 /* Copies out tick records with index >= first (at most max); *count receives how many. */
 int pdhg_dev_ticks(pdhg_dev* h, int64_t first, pdhg_tick* out, int64_t max, int64_t* count);
 

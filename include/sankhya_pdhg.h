@@ -46,6 +46,7 @@ void sankhya_model_free(sankhya_model* model);
 typedef struct {
   int32_t status; /* sankhya::SolveStatus */
   int64_t iterations, attempts, restarts, ticks, device_bytes;
+  int64_t fp32_iterations; /* mixed precision: iterations run before the FP64 continuation */
   double objective, dual_bound, solve_seconds, setup_seconds, loop_seconds, spectral_norm;
   double primal_infeasibility, dual_infeasibility, complementarity_violation, relative_gap;
   double final_primal, final_dual, final_gap;

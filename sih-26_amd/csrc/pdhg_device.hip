@@ -39,6 +39,11 @@
 
 namespace sankhya_hip {
 
+// Launch bounds of the two iteration kernels (a build-time knob for occupancy sweeps).
+#ifndef SANKHYA_ITEM_BOUNDS
+#define SANKHYA_ITEM_BOUNDS __launch_bounds__(kThreads)
+#endif
+
 constexpr int64_t kEvalInterval = 40;  // kEvaluationInterval, src/pdhg/pdhg.cpp:65
 constexpr int kDefaultChunk = 48;
 constexpr int64_t kTickCap = 1 << 16;
@@ -97,13 +102,24 @@ struct PrimalEpi {
   Iter<X> it;
   int cur, pend;
   double tau, omega;
-  __device__ void operator()(int j, const RowSum<1>& s, Acc<1, 0>& a) const {
-    const double x = static_cast<double>(it.xbuf[2 * (int64_t)j + cur]);
-    if (pend) it.xsum[j] += x;
-    const double grad = it.chat[j] + s.v[0];
-    const X stored = static_cast<X>(project(x - tau * grad, it.lhat[j], it.uhat[j]));
+  struct In {
+    double x, xs, c, l, u;
+  };
+  __device__ In load(int j) const {
+    In r;
+    r.x = static_cast<double>(it.xbuf[2 * (int64_t)j + cur]);
+    r.xs = pend ? it.xsum[j] : 0.0;
+    r.c = it.chat[j];
+    r.l = it.lhat[j];
+    r.u = it.uhat[j];
+    return r;
+  }
+  __device__ void operator()(int j, const RowSum<1>& s, const In& in, Acc<1, 0>& a) const {
+    if (pend) it.xsum[j] = in.xs + in.x;
+    const double grad = in.c + s.v[0];
+    const X stored = static_cast<X>(project(in.x - tau * grad, in.l, in.u));
     it.xbuf[2 * (int64_t)j + (cur ^ 1)] = stored;
-    const double d = static_cast<double>(stored) - x;
+    const double d = static_cast<double>(stored) - in.x;
     a.s[0] += 0.5 * omega * d * d;
   }
 };
@@ -128,13 +144,23 @@ struct DualEpi {
   Iter<X> it;
   int cur, pend;
   double sigma, omega;
-  __device__ void operator()(int i, const RowSum<2>& s, Acc<2, 0>& a) const {
-    const double y = static_cast<double>(it.ybuf[2 * (int64_t)i + cur]);
-    if (pend) it.ysum[i] += y;
-    const double v = y + sigma * s.v[0];
-    const X stored = static_cast<X>(v - sigma * project(v / sigma, it.rlhat[i], it.ruhat[i]));
+  struct In {
+    double y, ys, rl, ru;
+  };
+  __device__ In load(int i) const {
+    In r;
+    r.y = static_cast<double>(it.ybuf[2 * (int64_t)i + cur]);
+    r.ys = pend ? it.ysum[i] : 0.0;
+    r.rl = it.rlhat[i];
+    r.ru = it.ruhat[i];
+    return r;
+  }
+  __device__ void operator()(int i, const RowSum<2>& s, const In& in, Acc<2, 0>& a) const {
+    if (pend) it.ysum[i] = in.ys + in.y;
+    const double v = in.y + sigma * s.v[0];
+    const X stored = static_cast<X>(v - sigma * project(v / sigma, in.rl, in.ru));
     it.ybuf[2 * (int64_t)i + (cur ^ 1)] = stored;
-    const double dy = static_cast<double>(stored) - y;
+    const double dy = static_cast<double>(stored) - in.y;
     a.s[0] += 0.5 * dy * dy / omega;
     a.s[1] += dy * s.v[1];
   }
@@ -201,13 +227,24 @@ struct TickRowsEpi {
   Iter<X> it;
   int cur, tp;
   double count;
-  __device__ void operator()(int i, const RowSum<2>& s, Acc<7, 2>& a) const {
-    const double dr = it.dr[i];
-    const double yh = static_cast<double>(it.ybuf[2 * (int64_t)i + cur]);
-    const double yah = (tp ? it.ysum[i] + yh : it.ysum[i]) / count;
-    row_terms(s.v[0] / dr, yh * dr, it.rl[i], it.ru[i], a.s[0], a.s[1], a.s[2], a.m[0]);
-    row_terms(s.v[1] / dr, yah * dr, it.rl[i], it.ru[i], a.s[3], a.s[4], a.s[5], a.m[1]);
-    const double d = yh - it.yrest[i];
+  struct In {
+    double dr, yh, ys, rl, ru, yr;
+  };
+  __device__ In load(int i) const {
+    In r;
+    r.dr = it.dr[i];
+    r.yh = static_cast<double>(it.ybuf[2 * (int64_t)i + cur]);
+    r.ys = it.ysum[i];
+    r.rl = it.rl[i];
+    r.ru = it.ru[i];
+    r.yr = it.yrest[i];
+    return r;
+  }
+  __device__ void operator()(int i, const RowSum<2>& s, const In& in, Acc<7, 2>& a) const {
+    const double yah = (tp ? in.ys + in.yh : in.ys) / count;
+    row_terms(s.v[0] / in.dr, in.yh * in.dr, in.rl, in.ru, a.s[0], a.s[1], a.s[2], a.m[0]);
+    row_terms(s.v[1] / in.dr, yah * in.dr, in.rl, in.ru, a.s[3], a.s[4], a.s[5], a.m[1]);
+    const double d = in.yh - in.yr;
     a.s[6] += d * d;
   }
 };
@@ -231,14 +268,27 @@ struct TickColsEpi {
   Iter<X> it;
   int cur, tp;
   double count;
-  __device__ void operator()(int j, const RowSum<2>& s, Acc<7, 2>& a) const {
-    const double dc = it.dc[j];
-    const double c = it.cost[j];
-    const double xh = static_cast<double>(it.xbuf[2 * (int64_t)j + cur]);
-    const double xah = (tp ? it.xsum[j] + xh : it.xsum[j]) / count;
-    col_terms(c + s.v[0] / dc, xh * dc, it.cl[j], it.cu[j], c, a.s[0], a.s[1], a.s[2], a.m[0]);
-    col_terms(c + s.v[1] / dc, xah * dc, it.cl[j], it.cu[j], c, a.s[3], a.s[4], a.s[5], a.m[1]);
-    const double d = xh - it.xrest[j];
+  struct In {
+    double dc, c, xh, xs, cl, cu, xr;
+  };
+  __device__ In load(int j) const {
+    In r;
+    r.dc = it.dc[j];
+    r.c = it.cost[j];
+    r.xh = static_cast<double>(it.xbuf[2 * (int64_t)j + cur]);
+    r.xs = it.xsum[j];
+    r.cl = it.cl[j];
+    r.cu = it.cu[j];
+    r.xr = it.xrest[j];
+    return r;
+  }
+  __device__ void operator()(int j, const RowSum<2>& s, const In& in, Acc<7, 2>& a) const {
+    const double xah = (tp ? in.xs + in.xh : in.xs) / count;
+    col_terms(in.c + s.v[0] / in.dc, in.xh * in.dc, in.cl, in.cu, in.c, a.s[0], a.s[1], a.s[2],
+              a.m[0]);
+    col_terms(in.c + s.v[1] / in.dc, xah * in.dc, in.cl, in.cu, in.c, a.s[3], a.s[4], a.s[5],
+              a.m[1]);
+    const double d = in.xh - in.xr;
     a.s[6] += d * d;
   }
 };
@@ -259,10 +309,14 @@ struct SingleRowsEpi {
   Iter<X> it;
   const double* by;
   double* act_out;
-  __device__ void operator()(int i, const RowSum<1>& s, Acc<3, 1>& a) const {
-    const double act = s.v[0] / it.dr[i];
+  struct In {
+    double dr, y, rl, ru;
+  };
+  __device__ In load(int i) const { return In{it.dr[i], by[i], it.rl[i], it.ru[i]}; }
+  __device__ void operator()(int i, const RowSum<1>& s, const In& in, Acc<3, 1>& a) const {
+    const double act = s.v[0] / in.dr;
     if (act_out) act_out[i] = act;
-    row_terms(act, by[i], it.rl[i], it.ru[i], a.s[0], a.s[1], a.s[2], a.m[0]);
+    row_terms(act, in.y, in.rl, in.ru, a.s[0], a.s[1], a.s[2], a.m[0]);
   }
 };
 template <typename X>
@@ -280,10 +334,14 @@ struct SingleColsEpi {
   Iter<X> it;
   const double* bx;
   double* red_out;
-  __device__ void operator()(int j, const RowSum<1>& s, Acc<3, 1>& a) const {
-    const double d = it.cost[j] + s.v[0] / it.dc[j];
+  struct In {
+    double c, dc, x, cl, cu;
+  };
+  __device__ In load(int j) const { return In{it.cost[j], it.dc[j], bx[j], it.cl[j], it.cu[j]}; }
+  __device__ void operator()(int j, const RowSum<1>& s, const In& in, Acc<3, 1>& a) const {
+    const double d = in.c + s.v[0] / in.dc;
     if (red_out) red_out[j] = d;
-    col_terms(d, bx[j], it.cl[j], it.cu[j], it.cost[j], a.s[0], a.s[1], a.s[2], a.m[0]);
+    col_terms(d, in.x, in.cl, in.cu, in.c, a.s[0], a.s[1], a.s[2], a.m[0]);
   }
 };
 
@@ -300,7 +358,9 @@ struct PlainGather {
 };
 struct PlainEpi {
   double* y;
-  __device__ void operator()(int i, const RowSum<1>& s, Acc<1, 0>& a) const {
+  struct In {};
+  __device__ In load(int) const { return In{}; }
+  __device__ void operator()(int i, const RowSum<1>& s, const In&, Acc<1, 0>& a) const {
     y[i] = s.v[0];
     a.s[0] += s.v[0] * s.v[0];
   }
@@ -321,16 +381,17 @@ __device__ inline double tick_count(const Ctl* ctl) {
 // ---- iteration kernels ------------------------------------------------------------------
 
 template <typename V, typename X>
-__global__ __launch_bounds__(kThreads) void k_primal(PlanView<V> T, Iter<X> it, Ctl* ctl,
+__global__ SANKHYA_ITEM_BOUNDS void k_primal(PlanView<V> T, Iter<X> it, Ctl* ctl,
                                                      double* part, int force) {
   __shared__ double lds[kTile];
   __shared__ double red[4 * 2];
+  const int4 item = T.items[blockIdx.x];
   if (!force && ctl->halt) return;
   PrimalGather<X> gather{it.ybuf, ctl->cur};
   PrimalEpi<X> epi = primal_epi(it, ctl);
   Acc<1, 0> acc;
   acc.zero();
-  run_item<V, 1>(T, gather, epi, acc, lds, red);
+  run_item<V, 1>(T, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = acc.s[0];
 }
@@ -351,16 +412,17 @@ __global__ __launch_bounds__(kTail) void k_primal_tail(PlanView<V> T, Iter<X> it
 }
 
 template <typename V, typename X>
-__global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ctl* ctl,
+__global__ SANKHYA_ITEM_BOUNDS void k_dual(PlanView<V> A, Iter<X> it, Ctl* ctl,
                                                    double* part, int force) {
-  __shared__ double lds[2 * kTile];
+  __shared__ double lds[kTile];
   __shared__ double red[4 * 2];
+  const int4 item = A.items[blockIdx.x];
   if (!force && ctl->halt) return;
   DualGather<X> gather{it.xbuf, ctl->cur};
   DualEpi<X> epi = dual_epi(it, ctl);
   Acc<2, 0> acc;
   acc.zero();
-  run_item<V, 2>(A, gather, epi, acc, lds, red);
+  run_item<V, 2>(A, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = acc.s[0];
@@ -368,24 +430,35 @@ __global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ct
   }
 }
 
+// What the step rule reads from the control block, loaded by the deciding thread at kernel
+// start so that read shares the kernel's first memory round trip.
+struct StepSnap {
+  int64_t iteration, iteration_limit, averaged, attempts;
+  double eta, eta_ceiling;
+  int32_t cur;
+};
+__device__ inline StepSnap step_snap(const Ctl* ctl) {
+  return StepSnap{ctl->iteration, ctl->iteration_limit, ctl->averaged, ctl->attempts,
+                  ctl->eta, ctl->eta_ceiling, ctl->cur};
+}
+
 // Adaptive step (src/pdhg/pdhg.cpp:293-335) and the tick test (:341-342), one thread.
-__device__ void step_decide(Ctl* ctl, double mx, double my, double inter) {
+__device__ void step_decide(Ctl* ctl, const StepSnap& sn, double mx, double my, double inter) {
   const double movement = mx + my;
   const double interaction = fabs(inter);
   const bool no_info = interaction <= 0.0;
   const double limit = no_info ? __builtin_huge_val() : movement / interaction;
-  const int64_t iteration = ctl->iteration;
+  const int64_t iteration = sn.iteration;
   const double exponent = static_cast<double>(iteration + 1 > 2 ? iteration + 1 : 2);
   const double shrink = 1.0 - pow(exponent, -0.3);
   const double grow = 1.0 + pow(exponent, -0.6);
-  const double proposed = smin(shrink * limit, grow * ctl->eta);
-  const double eta = ctl->eta;
+  const double proposed = smin(shrink * limit, grow * sn.eta);
   int64_t it = iteration;
-  if (eta <= limit) {
-    ctl->cur ^= 1;
+  if (sn.eta <= limit) {
+    ctl->cur = sn.cur ^ 1;
     ++it;
     ctl->iteration = it;
-    ctl->averaged += 1;
+    ctl->averaged = sn.averaged + 1;
     ctl->pending = 1;
   } else {
     ctl->pending = 0;
@@ -393,16 +466,16 @@ __device__ void step_decide(Ctl* ctl, double mx, double my, double inter) {
   if (!no_info) {
     double e = proposed;
     if (e < 1e-12) e = 1e-12;
-    if (ctl->eta_ceiling < e) e = ctl->eta_ceiling;
+    if (sn.eta_ceiling < e) e = sn.eta_ceiling;
     ctl->eta = e;
   }
-  ctl->attempts += 1;
+  ctl->attempts = sn.attempts + 1;
   ctl->no_info = no_info ? 1 : 0;
   const bool tick = it != 0 && (it % kEvalInterval == 0 || no_info);
   if (tick) {
     ctl->tick = 1;
     ctl->halt = 1;
-  } else if (it >= ctl->iteration_limit) {
+  } else if (it >= sn.iteration_limit) {
     ctl->halt = 1;
     ctl->limit_hit = 1;
   }
@@ -419,6 +492,8 @@ __global__ __launch_bounds__(kTail) void k_dual_tail(PlanView<V> A, Iter<X> it, 
   __shared__ double red[16 * 5];
   if (!force && ctl->halt) return;
   DualEpi<X> epi = dual_epi(it, ctl);
+  StepSnap snap{};
+  if (threadIdx.x == 0) snap = step_snap(ctl);
   // s[0] primal movement, s[1] dual movement, s[2] interaction (per-workgroup partials);
   // s[3], s[4] the same two dual terms from the long rows.
   Acc<5, 0> a;
@@ -438,14 +513,15 @@ __global__ __launch_bounds__(kTail) void k_dual_tail(PlanView<V> A, Iter<X> it, 
   a.s[3] = lacc.s[0];
   a.s[4] = lacc.s[1];
   block_reduce(a, red);
-  if (threadIdx.x == 0) step_decide(ctl, a.s[0], a.s[1] + a.s[3], a.s[2] + a.s[4]);
+  if (threadIdx.x == 0) step_decide(ctl, snap, a.s[0], a.s[1] + a.s[3], a.s[2] + a.s[4]);
 }
 
 template <typename V, typename X>
 __global__ __launch_bounds__(kThreads) void k_eval_rows(PlanView<V> A, Iter<X> it, Ctl* ctl,
                                                         double* part) {
-  __shared__ double lds[2 * kTile];
+  __shared__ double lds[kTile];
   __shared__ double red[4 * 9];
+  const int4 item = A.items[blockIdx.x];
   if (!ctl->tick) return;
   const int cur = ctl->cur, tp = ctl->pending;
   const double count = tick_count(ctl);
@@ -453,7 +529,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_rows(PlanView<V> A, Iter<X> i
   TickRowsEpi<X> epi{it, cur, tp, count};
   Acc<7, 2> acc;
   acc.zero();
-  run_item<V, 2>(A, gather, epi, acc, lds, red);
+  run_item<V, 2>(A, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) {
     for (int k = 0; k < 7; ++k) part[9 * blockIdx.x + k] = acc.s[k];
@@ -464,8 +540,9 @@ __global__ __launch_bounds__(kThreads) void k_eval_rows(PlanView<V> A, Iter<X> i
 template <typename V, typename X>
 __global__ __launch_bounds__(kThreads) void k_eval_cols(PlanView<V> T, Iter<X> it, Ctl* ctl,
                                                         double* part) {
-  __shared__ double lds[2 * kTile];
+  __shared__ double lds[kTile];
   __shared__ double red[4 * 9];
+  const int4 item = T.items[blockIdx.x];
   if (!ctl->tick) return;
   const int cur = ctl->cur, tp = ctl->pending;
   const double count = tick_count(ctl);
@@ -473,7 +550,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_cols(PlanView<V> T, Iter<X> i
   TickColsEpi<X> epi{it, cur, tp, count};
   Acc<7, 2> acc;
   acc.zero();
-  run_item<V, 2>(T, gather, epi, acc, lds, red);
+  run_item<V, 2>(T, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) {
     for (int k = 0; k < 7; ++k) part[9 * blockIdx.x + k] = acc.s[k];
@@ -652,6 +729,16 @@ __global__ __launch_bounds__(kThreads) void k_apply(Iter<X> it, Ctl* ctl) {
   }
 }
 
+// Exact FP32 -> FP64 widening of the iterate pairs (the mixed-precision handoff).
+__global__ void k_widen(const float* xs, double* xd, int64_t nx, const float* ys, double* yd,
+                        int64_t ny) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nx; k += stride)
+    xd[k] = static_cast<double>(xs[k]);
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ny; k += stride)
+    yd[k] = static_cast<double>(ys[k]);
+}
+
 __global__ void k_clear(Ctl* ctl) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     ctl->apply = 0;
@@ -689,11 +776,12 @@ __global__ __launch_bounds__(kThreads) void k_single_rows(PlanView<V> A, Iter<X>
                                                           double* act_out, double* part) {
   __shared__ double lds[kTile];
   __shared__ double red[4 * 4];
+  const int4 item = A.items[blockIdx.x];
   SingleRowsGather<X> gather{it, bx};
   SingleRowsEpi<X> epi{it, by, act_out};
   Acc<3, 1> acc;
   acc.zero();
-  run_item<V, 1>(A, gather, epi, acc, lds, red);
+  run_item<V, 1>(A, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) {
     for (int k = 0; k < 3; ++k) part[4 * blockIdx.x + k] = acc.s[k];
@@ -707,11 +795,12 @@ __global__ __launch_bounds__(kThreads) void k_single_cols(PlanView<V> T, Iter<X>
                                                           double* red_out, double* part) {
   __shared__ double lds[kTile];
   __shared__ double red[4 * 4];
+  const int4 item = T.items[blockIdx.x];
   SingleColsGather<X> gather{it, by};
   SingleColsEpi<X> epi{it, bx, red_out};
   Acc<3, 1> acc;
   acc.zero();
-  run_item<V, 1>(T, gather, epi, acc, lds, red);
+  run_item<V, 1>(T, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) {
     for (int k = 0; k < 3; ++k) part[4 * blockIdx.x + k] = acc.s[k];
@@ -768,12 +857,13 @@ __global__ __launch_bounds__(kThreads) void k_spmv(PlanView<V> P, const double* 
                                                    const Ctl* scale, double* part) {
   __shared__ double lds[kTile];
   __shared__ double red[4];
+  const int4 item = P.items[blockIdx.x];
   if (scale && scale->pw_zero) return;
   PlainGather gather{x, 0.0, scale ? scale->pw_len : 0.0};
   PlainEpi epi{y};
   Acc<1, 0> acc;
   acc.zero();
-  run_item<V, 1>(P, gather, epi, acc, lds, red);
+  run_item<V, 1>(P, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = acc.s[0];
 }
@@ -814,12 +904,11 @@ struct DevPlan {
   int32_t rows = 0, nitems = 0, nlr = 0, nseg = 0;
   int32_t* rowptr = nullptr;
   int32_t* col = nullptr;
-  void* val = nullptr;
+  float* val32 = nullptr;   // FP32 values (mixed precision only)
+  double* val64 = nullptr;  // FP64 values (always: ticks and reports evaluate in FP64)
   int4* items = nullptr;
-  int32_t* lr_nseg = nullptr;
-  int32_t* lr_base = nullptr;
-  int32_t* lr_row = nullptr;
-  int2* lr_groups = nullptr;
+  int4* lr_info = nullptr;
+  int4* lr_groups = nullptr;
   int32_t ngroups = 0;
   double* seg_part = nullptr;
   template <typename V>
@@ -827,11 +916,10 @@ struct DevPlan {
     PlanView<V> p;
     p.rowptr = rowptr;
     p.col = col;
-    p.val = static_cast<const V*>(val);
+    if constexpr (sizeof(V) == 8) p.val = val64;
+    else p.val = val32;
     p.items = items;
-    p.lr_nseg = lr_nseg;
-    p.lr_base = lr_base;
-    p.lr_row = lr_row;
+    p.lr_info = lr_info;
     p.lr_groups = lr_groups;
     p.seg_part = seg_part;
     p.nitems = nitems;
@@ -844,7 +932,8 @@ struct DevPlan {
 struct HostPlan {
   std::vector<int4> items;
   std::vector<int32_t> lr_nseg, lr_base, lr_row;
-  std::vector<int2> groups;  // consecutive long rows whose segments fit one LDS round
+  std::vector<int4> groups;  // {first, last) long rows, first segment, segment count}: rows
+                             // whose segment sums fit one LDS round, at most kTail of them
   int32_t nseg = 0;
 };
 
@@ -925,8 +1014,11 @@ HostPlan build_plan(int32_t rows, int32_t ncols, const int32_t* rowptr, const in
   const int32_t nlr = static_cast<int32_t>(hp.lr_nseg.size());
   for (int32_t lr = 0; lr < nlr;) {
     int32_t end = lr + 1;
-    while (end < nlr && hp.lr_base[end] + hp.lr_nseg[end] - hp.lr_base[lr] <= kChunk) ++end;
-    hp.groups.push_back(make_int2(lr, end));
+    while (end < nlr && end - lr < kTail &&
+           hp.lr_base[end] + hp.lr_nseg[end] - hp.lr_base[lr] <= kChunk)
+      ++end;
+    hp.groups.push_back(make_int4(lr, end, hp.lr_base[lr],
+                                  hp.lr_base[end - 1] + hp.lr_nseg[end - 1] - hp.lr_base[lr]));
     lr = end;
   }
   return hp;
@@ -945,8 +1037,9 @@ struct pdhg_dev {
   int32_t m = 0, n = 0;
   int64_t nnz = 0;
   DevPlan A, T;
-  void* xbuf = nullptr;
-  void* ybuf = nullptr;
+  int arith = PDHG_PRECISION_FP64;  // the iterates' current precision (mixed: FP32 until promoted)
+  float *xb32 = nullptr, *yb32 = nullptr;    // iterate pairs, FP32 phase
+  double *xb64 = nullptr, *yb64 = nullptr;   // iterate pairs, FP64
   double *xsum = nullptr, *ysum = nullptr, *xrest = nullptr, *yrest = nullptr;
   double *bestx = nullptr, *besty = nullptr, *sx = nullptr, *sy = nullptr, *sx2 = nullptr;
   double* sact = nullptr;
@@ -959,6 +1052,7 @@ struct pdhg_dev {
   Ctl* ctl_host = nullptr;
   pdhg_tick* ticks = nullptr;
   hipGraphExec_t exec = nullptr;
+  int exec_arith = -1;  // precision the captured graph was built for
   int chunk = kDefaultChunk;
   bool uploaded = false, started = false;
   int64_t bytes = 0;
@@ -1041,6 +1135,16 @@ bool plan_ok(const HostPlan& hp, const std::vector<int4>& logical, int32_t rows,
   // A single long row must fit one LDS round of the tail kernels (kChunk segment sums).
   for (const int32_t ns : hp.lr_nseg)
     if (ns > kChunk) return false;
+  // Tail groups: consecutive, covering every long row, at most one row per tail lane and at
+  // most kChunk segment sums each.
+  int32_t next_lr = 0;
+  for (const int4& g : hp.groups) {
+    if (g.x != next_lr || g.y <= g.x || g.y - g.x > kTail || g.w > kChunk) return false;
+    if (g.z != hp.lr_base[g.x] || g.w != hp.lr_base[g.y - 1] + hp.lr_nseg[g.y - 1] - g.z)
+      return false;
+    next_lr = g.y;
+  }
+  if (next_lr != static_cast<int32_t>(hp.lr_nseg.size())) return false;
   return next_row == rows && next_k < 0;
 }
 
@@ -1069,22 +1173,20 @@ int make_plan(pdhg_dev* h, DevPlan* P, int32_t rows, int32_t ncols, int64_t nnz,
   if (h->precision == PDHG_PRECISION_MIXED) {
     std::vector<float> f(un + kPad, 0.0f);
     for (int64_t k = 0; k < nnz; ++k) f[k] = static_cast<float>(val[k]);
-    float* d = nullptr;
-    if ((rc = upload(h, &d, f.data(), f.size()))) return rc;
+    if ((rc = upload(h, &P->val32, f.data(), f.size()))) return rc;
     HIP_TRY(h, hipStreamSynchronize(h->stream));
-    P->val = d;
-  } else {
+  }
+  {
     std::vector<double> v(un + kPad, 0.0);
     std::copy(val, val + nnz, v.begin());
-    double* d = nullptr;
-    if ((rc = upload(h, &d, v.data(), v.size()))) return rc;
+    if ((rc = upload(h, &P->val64, v.data(), v.size()))) return rc;
     HIP_TRY(h, hipStreamSynchronize(h->stream));
-    P->val = d;
   }
   if ((rc = upload(h, &P->items, hp.items.data(), hp.items.size()))) return rc;
-  if ((rc = upload(h, &P->lr_nseg, hp.lr_nseg.data(), hp.lr_nseg.size()))) return rc;
-  if ((rc = upload(h, &P->lr_base, hp.lr_base.data(), hp.lr_base.size()))) return rc;
-  if ((rc = upload(h, &P->lr_row, hp.lr_row.data(), hp.lr_row.size()))) return rc;
+  std::vector<int4> info(hp.lr_nseg.size());
+  for (size_t k = 0; k < info.size(); ++k)
+    info[k] = make_int4(hp.lr_row[k], hp.lr_base[k], hp.lr_nseg[k], 0);
+  if ((rc = upload(h, &P->lr_info, info.data(), info.size()))) return rc;
   if ((rc = upload(h, &P->lr_groups, hp.groups.data(), hp.groups.size()))) return rc;
   P->ngroups = static_cast<int32_t>(hp.groups.size());
   if ((rc = dalloc(h, &P->seg_part, static_cast<size_t>(hp.nseg) * 2))) return rc;
@@ -1095,8 +1197,13 @@ int make_plan(pdhg_dev* h, DevPlan* P, int32_t rows, int32_t ncols, int64_t nnz,
 template <typename X>
 Iter<X> iter_of(const pdhg_dev* h) {
   Iter<X> it;
-  it.xbuf = static_cast<X*>(h->xbuf);
-  it.ybuf = static_cast<X*>(h->ybuf);
+  if constexpr (sizeof(X) == 8) {
+    it.xbuf = h->xb64;
+    it.ybuf = h->yb64;
+  } else {
+    it.xbuf = h->xb32;
+    it.ybuf = h->yb32;
+  }
   it.xsum = h->xsum;
   it.ysum = h->ysum;
   it.xrest = h->xrest;
@@ -1145,12 +1252,17 @@ template <typename V, typename X>
 void launch_chunk(pdhg_dev* h) {
   const Iter<X> it = iter_of<X>(h);
   for (int s = 0; s < h->chunk; ++s) launch_attempt<V, X>(h, 0);
+  // The evaluation tick reads the FP64 values in either precision: every convergence,
+  // restart and primal-weight decision is taken on FP64 residuals.
   if (h->A.nitems > 0)
-    k_eval_rows<V, X><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<V>(), it, h->ctl, h->part_er);
+    k_eval_rows<double, X><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<double>(), it, h->ctl,
+                                                                    h->part_er);
   if (h->T.nitems > 0)
-    k_eval_cols<V, X><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<V>(), it, h->ctl, h->part_ec);
-  k_decide<V, X><<<1, kTail, 0, h->stream>>>(h->A.view<V>(), h->T.view<V>(), it, h->ctl, h->part_er,
-                                             h->A.nitems, h->part_ec, h->T.nitems, h->ticks);
+    k_eval_cols<double, X><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<double>(), it, h->ctl,
+                                                                    h->part_ec);
+  k_decide<double, X><<<1, kTail, 0, h->stream>>>(h->A.view<double>(), h->T.view<double>(), it,
+                                                  h->ctl, h->part_er, h->A.nitems, h->part_ec,
+                                                  h->T.nitems, h->ticks);
   k_apply<X><<<apply_grid(h), kThreads, 0, h->stream>>>(it, h->ctl);
   k_clear<<<1, 64, 0, h->stream>>>(h->ctl);
 }
@@ -1179,7 +1291,27 @@ void launch_spmv(pdhg_dev* h, const DevPlan& P, const double* x, double* y, cons
   k_spmv_tail<V><<<1, kTail, 0, h->stream>>>(P.view<V>(), y, scale, lr_sum);
 }
 
-bool mixed(const pdhg_dev* h) { return h->precision == PDHG_PRECISION_MIXED; }
+// True while the iterates are FP32 (a mixed handle before pdhg_dev_promote).
+bool mixed(const pdhg_dev* h) { return h->arith == PDHG_PRECISION_MIXED; }
+
+// (Re)captures the graph chunk for the handle's current precision.
+int capture(pdhg_dev* h) {
+  if (h->exec) {
+    HIP_TRY(h, hipGraphExecDestroy(h->exec));
+    h->exec = nullptr;
+  }
+  hipGraph_t graph = nullptr;
+  HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  if (mixed(h)) launch_chunk<float, float>(h);
+  else launch_chunk<double, double>(h);
+  HIP_TRY(h, hipStreamEndCapture(h->stream, &graph));
+  const hipError_t e = hipGraphInstantiate(&h->exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess)
+    return fail(h, PDHG_E_LAUNCH, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  h->exec_arith = h->arith;
+  return PDHG_OK;
+}
 
 int check_ready(pdhg_dev* h, bool need_start) {
   if (!h) return PDHG_E_ARGUMENT;
@@ -1220,6 +1352,7 @@ int pdhg_dev_create(int device, int precision, pdhg_dev** out) {
   pdhg_dev* h = new pdhg_dev();
   h->device = device;
   h->precision = precision;
+  h->arith = precision;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return PDHG_E_LAUNCH;
@@ -1283,13 +1416,13 @@ int pdhg_dev_upload(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz, const int32_
   int rc;
   if ((rc = make_plan(h, &h->A, m, n, nnz, a_rowptr, a_colidx, a_val))) return rc;
   if ((rc = make_plan(h, &h->T, n, m, nnz, at_rowptr, at_colidx, at_val))) return rc;
-  const size_t xs = mixed(h) ? sizeof(float) : sizeof(double);
-  {
-    char* p = nullptr;
-    if ((rc = dalloc(h, &p, 2 * xs * std::max<int32_t>(n, 1)))) return rc;
-    h->xbuf = p;
-    if ((rc = dalloc(h, &p, 2 * xs * std::max<int32_t>(m, 1)))) return rc;
-    h->ybuf = p;
+  if (mixed(h)) {
+    if ((rc = dalloc(h, &h->xb32, 2 * static_cast<size_t>(n))) ||
+        (rc = dalloc(h, &h->yb32, 2 * static_cast<size_t>(m))))
+      return rc;
+  } else if ((rc = dalloc(h, &h->xb64, 2 * static_cast<size_t>(n))) ||
+             (rc = dalloc(h, &h->yb64, 2 * static_cast<size_t>(m)))) {
+    return rc;
   }
   const size_t un = static_cast<size_t>(n), um = static_cast<size_t>(m);
   double** nvec[] = {&h->xsum, &h->xrest, &h->bestx, &h->sx, &h->sx2};
@@ -1367,13 +1500,8 @@ int pdhg_dev_spectral_norm(pdhg_dev* h, const double* v0, int iterations, double
   HIP_TRY(h, hipMemcpyAsync(h->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
   HIP_TRY(h, hipMemcpyAsync(h->sx, v0, sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
   for (int step = 0; step < iterations; ++step) {
-    if (mixed(h)) {
-      launch_spmv<float>(h, h->A, h->sx, h->sy, h->ctl, h->part_er, h->lr_sum);
-      launch_spmv<float>(h, h->T, h->sy, h->sx, nullptr, h->part_ec, h->lr_sum);
-    } else {
-      launch_spmv<double>(h, h->A, h->sx, h->sy, h->ctl, h->part_er, h->lr_sum);
-      launch_spmv<double>(h, h->T, h->sy, h->sx, nullptr, h->part_ec, h->lr_sum);
-    }
+    launch_spmv<double>(h, h->A, h->sx, h->sy, h->ctl, h->part_er, h->lr_sum);
+    launch_spmv<double>(h, h->T, h->sy, h->sx, nullptr, h->part_ec, h->lr_sum);
     k_pw_finish<<<1, kThreads, 0, h->stream>>>(h->ctl, h->part_ec, h->T.nitems, h->lr_sum);
     HIP_TRY(h, hipGetLastError());
   }
@@ -1393,6 +1521,7 @@ int pdhg_dev_start(pdhg_dev* h, const pdhg_params* p) {
   if (rc) return rc;
   if (!p || p->iteration_limit < 0) return fail(h, PDHG_E_ARGUMENT, "bad parameters");
   HIP_TRY(h, hipSetDevice(h->device));
+  h->arith = h->precision;  // a mixed handle starts (again) in its FP32 phase
   Ctl c;
   HIP_TRY(h, hipMemcpyAsync(&c, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -1422,23 +1551,49 @@ int pdhg_dev_start(pdhg_dev* h, const pdhg_params* p) {
   HIP_TRY(h, hipGetLastError());
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   const int chunk = p->chunk_attempts > 0 ? p->chunk_attempts : kDefaultChunk;
-  if (!h->exec || chunk != h->chunk) {
-    if (h->exec) {
-      hipGraphExecDestroy(h->exec);
-      h->exec = nullptr;
-    }
+  if (!h->exec || chunk != h->chunk || h->exec_arith != h->arith) {
     h->chunk = chunk;
-    hipGraph_t graph = nullptr;
-    HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    if (mixed(h)) launch_chunk<float, float>(h);
-    else launch_chunk<double, double>(h);
-    HIP_TRY(h, hipStreamEndCapture(h->stream, &graph));
-    const hipError_t e = hipGraphInstantiate(&h->exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(h, PDHG_E_LAUNCH, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    if ((rc = capture(h))) return rc;
   }
   h->started = true;
   return PDHG_OK;
+}
+
+int pdhg_dev_promote(pdhg_dev* h, const pdhg_params* p) {
+  int rc = check_ready(h, true);
+  if (rc) return rc;
+  if (!p || p->iteration_limit < 0) return fail(h, PDHG_E_ARGUMENT, "bad parameters");
+  if (h->precision != PDHG_PRECISION_MIXED || !mixed(h))
+    return fail(h, PDHG_E_STATE, "promote needs a mixed-precision handle still in its FP32 phase");
+  HIP_TRY(h, hipSetDevice(h->device));
+  if (!h->xb64) {
+    if ((rc = dalloc(h, &h->xb64, 2 * static_cast<size_t>(h->n))) ||
+        (rc = dalloc(h, &h->yb64, 2 * static_cast<size_t>(h->m))))
+      return rc;
+  }
+  const int64_t w = std::max<int64_t>(std::max<int64_t>(2 * int64_t{h->n}, 2 * int64_t{h->m}), 1);
+  const int grid = static_cast<int>(std::min<int64_t>((w + kThreads - 1) / kThreads, 8192));
+  k_widen<<<grid, kThreads, 0, h->stream>>>(h->xb32, h->xb64, 2 * int64_t{h->n}, h->yb32, h->yb64,
+                                            2 * int64_t{h->m});
+  HIP_TRY(h, hipGetLastError());
+  Ctl c;
+  HIP_TRY(h, hipMemcpyAsync(&c, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  c.tolerance = p->tolerance;
+  c.stop_at_request = p->stop_at_request ? 1 : 0;
+  c.stop_rule = p->stop_rule == 1 ? 1 : 0;
+  c.iteration_limit = p->iteration_limit;
+  c.done = 0;
+  c.limit_hit = 0;
+  c.halt = 0;
+  if (c.iteration >= c.iteration_limit) {
+    c.halt = 1;
+    c.limit_hit = 1;
+  }
+  HIP_TRY(h, hipMemcpyAsync(h->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->arith = PDHG_PRECISION_FP64;
+  return capture(h);
 }
 
 int pdhg_dev_run(pdhg_dev* h, int chunks, pdhg_state* st) {
@@ -1487,8 +1642,7 @@ int pdhg_dev_ticks(pdhg_dev* h, int64_t first, pdhg_tick* out, int64_t max, int6
 int pdhg_dev_finish(pdhg_dev* h, double* col_value, double* y, double* reduced, double* residuals) {
   int rc = check_ready(h, true);
   if (rc) return rc;
-  if (mixed(h)) launch_single<float, float>(h, h->bestx, h->besty, nullptr, h->sx2);
-  else launch_single<double, double>(h, h->bestx, h->besty, nullptr, h->sx2);
+  launch_single<double, double>(h, h->bestx, h->besty, nullptr, h->sx2);
   HIP_TRY(h, hipGetLastError());
   if (col_value) HIP_TRY(h, hipMemcpyAsync(col_value, h->bestx, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));
   if (y) HIP_TRY(h, hipMemcpyAsync(y, h->besty, sizeof(double) * h->m, hipMemcpyDeviceToHost, h->stream));
@@ -1507,8 +1661,7 @@ int pdhg_dev_evaluate(pdhg_dev* h, const double* x, const double* y, double* act
   HIP_TRY(h, hipMemcpyAsync(h->sx, x, sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
   HIP_TRY(h, hipMemcpyAsync(h->sy, y, sizeof(double) * h->m, hipMemcpyHostToDevice, h->stream));
   double* act = h->sact;
-  if (mixed(h)) launch_single<float, float>(h, h->sx, h->sy, act, h->sx2);
-  else launch_single<double, double>(h, h->sx, h->sy, act, h->sx2);
+  launch_single<double, double>(h, h->sx, h->sy, act, h->sx2);
   HIP_TRY(h, hipGetLastError());
   if (activity) HIP_TRY(h, hipMemcpyAsync(activity, act, sizeof(double) * h->m, hipMemcpyDeviceToHost, h->stream));
   if (reduced) HIP_TRY(h, hipMemcpyAsync(reduced, h->sx2, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));

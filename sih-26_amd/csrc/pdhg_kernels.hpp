@@ -42,7 +42,10 @@ constexpr int kTile = SANKHYA_TILE;     // nonzeros per stream item
 constexpr int kSeg = SANKHYA_SEG;       // nonzeros per long-row segment
 constexpr int kTail = 1024;     // threads of the single-workgroup tail kernels
 constexpr int kChunk = 2048;    // long-row segment sums staged in LDS per tail round
-constexpr int kMaxRows = 1024;  // rows per stream item
+#ifndef SANKHYA_MAX_ROWS
+#define SANKHYA_MAX_ROWS 512
+#endif
+constexpr int kMaxRows = SANKHYA_MAX_ROWS;  // rows per stream item
 
 __device__ __forceinline__ bool finite_bound(double v) {
   // include/sankhya/types.hpp:89-96: absent when infinite or at/after the MPS 1e30 marker.
@@ -210,12 +213,10 @@ struct PlanView {
   const int32_t* rowptr;
   const int32_t* col;
   const V* val;
-  const int4* items;  // {r0, r1, k0, k1}; r1 < 0: a long-row segment {seg id, -lr-1, k0, k1}
-  const int32_t* lr_nseg;
-  const int32_t* lr_base;
-  const int32_t* lr_row;    // the row of each long row
-  const int2* lr_groups;    // [first, last) long rows whose segments fit one LDS round
-  double* seg_part;         // segment sums, NR per segment
+  const int4* items;   // {r0, r1, k0, k1}; r1 < 0: a long-row segment {seg id, -lr-1, k0, k1}
+  const int4* lr_info;  // per long row {row, first segment, segment count, 0}
+  const int4* lr_groups;  // {first, last) long rows, first segment, segment count} per LDS round
+  double* seg_part;      // segment sums, NR per segment
   int32_t nitems;
   int32_t nlr;
   int32_t ngroups;
@@ -226,17 +227,28 @@ struct RowSum {
   double v[NR];
 };
 
-// Process this workgroup's item. gather(j) returns RowSum<NR> for column j; epi(row, sum,
-// acc) runs once per completed row (in one thread). A long-row segment only stores its sum;
-// finish_long_rows completes those rows later.
+// Rows one lane finishes in a stream item, at most: a lane group of L lanes owns rows
+// grp, grp + ngrp, ...; with L > 1 an item has at most ngrp rows, with L = 1 at most kMaxRows.
+constexpr int kRowsPerLane = kMaxRows / kThreads;
+
+// Process one item. gather(j) returns RowSum<NR> for column j. The epilogue type E provides
+// E::In load(row) (the row's own inputs: iterate, bounds, running sum) and
+// operator()(row, sum, in, acc), run once per completed row in one lane. A long-row segment
+// only stores its sum; finish_long_rows completes those rows later.
+//
+// Memory round trips: the item descriptor arrives with the caller's control-block loads; then
+// ONE round trip brings the item's nonzeros, its row extents and every epilogue input; then the
+// gathers. Products are staged in LDS and each row is summed by L lanes with shuffles.
 template <typename V, int NR, int NS, int NM, class G, class E>
-__device__ void run_item(const PlanView<V>& P, G& gather, E& epi, Acc<NS, NM>& acc,
-                         double* lds, double* red) {
-  const int4 it = P.items[blockIdx.x];
+__device__ void run_item(const PlanView<V>& P, const int4 it, G& gather, E& epi,
+                         Acc<NS, NM>& acc, double* lds, double* red) {
   const int tid = threadIdx.x;
   if (it.y >= 0) {
     const int r0 = it.x, r1 = it.y, k0 = it.z, k1 = it.w;
-    const int cnt = k1 - k0;
+    const int rows = r1 - r0;
+    int L = 64;
+    while (L > 1 && rows * L > kThreads) L >>= 1;
+    const int grp = tid / L, lane = tid & (L - 1), ngrp = kThreads / L;
     // Two nonzeros per lane per step (one 16-byte value load, one 8-byte index load), from a
     // pair-aligned base; kTile / (2 kThreads) steps, all loads issued before the first gather.
     const int kb = k0 & ~1;
@@ -251,42 +263,80 @@ __device__ void run_item(const PlanView<V>& P, G& gather, E& epi, Acc<NS, NM>& a
         aval[u] = load_vals2(P.val + k);
       }
     }
+    int rs[kRowsPerLane], re[kRowsPerLane];
+    typename E::In ein[kRowsPerLane];
+#pragma unroll
+    for (int q = 0; q < kRowsPerLane; ++q) {
+      rs[q] = 0;
+      re[q] = 0;
+      const int rr = grp + q * ngrp;
+      if (lane == 0 && rr < rows) {
+        rs[q] = P.rowptr[r0 + rr];
+        re[q] = P.rowptr[r0 + rr + 1];
+        ein[q] = epi.load(r0 + rr);
+      }
+    }
+    // Products of right-hand side 0 go to LDS; those of the others wait in registers and
+    // reuse the same kTile doubles of LDS one after another (NR passes, one buffer).
+    double hold[NR > 1 ? NR - 1 : 1][U][2];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = kb + 2 * (u * kThreads + tid);
       if (k < k1) {
         if (k >= k0) {
           const RowSum<NR> g = gather(cidx[u].x);
+          lds[k - k0] = aval[u].x * g.v[0];
 #pragma unroll
-          for (int r = 0; r < NR; ++r) lds[r * kTile + (k - k0)] = aval[u].x * g.v[r];
+          for (int r = 1; r < NR; ++r) hold[r - 1][u][0] = aval[u].x * g.v[r];
         }
         if (k + 1 < k1) {
           const RowSum<NR> g = gather(cidx[u].y);
+          lds[k + 1 - k0] = aval[u].y * g.v[0];
 #pragma unroll
-          for (int r = 0; r < NR; ++r) lds[r * kTile + (k + 1 - k0)] = aval[u].y * g.v[r];
+          for (int r = 1; r < NR; ++r) hold[r - 1][u][1] = aval[u].y * g.v[r];
         }
       }
     }
-    __syncthreads();
-    const int rows = r1 - r0;
-    int L = 64;
-    while (L > 1 && rows * L > kThreads) L >>= 1;
-    const int grp = tid / L, lane = tid & (L - 1), ngrp = kThreads / L;
-    for (int rr = grp; rr < rows; rr += ngrp) {
-      const int row = r0 + rr;
-      const int s0 = P.rowptr[row] - k0, s1 = P.rowptr[row + 1] - k0;
-      RowSum<NR> sum;
+    const int src = (tid & 63) & ~(L - 1);  // the group's first lane within the wavefront
 #pragma unroll
-      for (int r = 0; r < NR; ++r) sum.v[r] = 0.0;
-      for (int s = s0 + lane; s < s1; s += L) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) sum.v[r] += lds[r * kTile + s];
+    for (int q = 0; q < kRowsPerLane; ++q) {
+      if (L > 1) {
+        rs[q] = __shfl(rs[q], src);
+        re[q] = __shfl(re[q], src);
       }
-      for (int off = L >> 1; off > 0; off >>= 1) {
+      rs[q] -= k0;
+      re[q] -= k0;
+    }
+    RowSum<NR> sum[kRowsPerLane];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) sum.v[r] += __shfl_xor(sum.v[r], off);
+    for (int r = 0; r < NR; ++r) {
+      if (r > 0) {
+        __syncthreads();  // every lane has read pass r-1
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = kb + 2 * (u * kThreads + tid);
+          if (k < k1) {
+            if (k >= k0) lds[k - k0] = hold[r - 1][u][0];
+            if (k + 1 < k1) lds[k + 1 - k0] = hold[r - 1][u][1];
+          }
+        }
       }
-      if (lane == 0) epi(row, sum, acc);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < kRowsPerLane; ++q) {
+        const int rr = grp + q * ngrp;
+        double t = 0.0;
+        if (rr < rows) {
+          for (int s = rs[q] + lane; s < re[q]; s += L) t += lds[s];
+          for (int off = L >> 1; off > 0; off >>= 1) t += __shfl_xor(t, off);
+        }
+        sum[q].v[r] = t;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kRowsPerLane; ++q) {
+      const int rr = grp + q * ngrp;
+      if (rr < rows && lane == 0) epi(r0 + rr, sum[q], ein[q], acc);
     }
     return;
   }
@@ -334,10 +384,11 @@ __device__ void run_item(const PlanView<V>& P, G& gather, E& epi, Acc<NS, NM>& a
 }
 
 // Completes every long row of P in one workgroup (launched after the kernel that wrote the
-// segment sums): for each group of long rows, stage their segment sums in LDS with coalesced
-// loads, then a power-of-two group of lanes per row adds its segments in index order and one
-// lane runs epi. `pre()` runs once, between issuing the first group's loads and using them, so
-// a caller's own independent loads share that memory round trip.
+// segment sums): for each group of at most blockDim long rows, stage their segment sums in
+// LDS with coalesced loads while each row's lane fetches the row's epilogue inputs; then a
+// power-of-two group of lanes per row adds its segments in index order and one lane runs epi.
+// `pre()` runs once, after the first group's loads are issued and before they are used, so a
+// caller's own independent loads share that memory round trip.
 struct NoPre {
   __device__ void operator()() const {}
 };
@@ -347,9 +398,15 @@ __device__ void finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
                                  double* lds /* kChunk * NR */, Pre pre = Pre()) {
   bool pre_done = false;
   for (int g = 0; g < P.ngroups; ++g) {
-    const int2 grp = P.lr_groups[g];
-    const int base0 = P.lr_base[grp.x];
-    const int cnt = P.lr_base[grp.y - 1] + P.lr_nseg[grp.y - 1] - base0;
+    const int4 grp = P.lr_groups[g];  // {first, last, base0, segment count}
+    const int nrows = grp.y - grp.x, base0 = grp.z, cnt = grp.w;
+    int L = 64;
+    while (L > 1 && nrows * L > static_cast<int>(blockDim.x)) L >>= 1;
+    const int gi = threadIdx.x / L, lane = threadIdx.x & (L - 1);
+    const int lr = grp.x + gi;  // at most one row per lane group: nrows <= blockDim / L
+    const bool mine = gi < nrows;
+    int4 info = make_int4(0, 0, 0, 0);
+    if (mine) info = P.lr_info[lr];
     // kChunk * NR / kTail loads per lane at most, held in registers until `pre` has run.
     constexpr int U = (kChunk * NR + kTail - 1) / kTail;
     double v[U];
@@ -358,6 +415,8 @@ __device__ void finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
       const int t = u * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
       v[u] = t < cnt * NR ? P.seg_part[(int64_t)base0 * NR + t] : 0.0;
     }
+    typename E::In ein;
+    if (mine && lane == 0) ein = epi.load(info.x);
     if (!pre_done) {
       pre();
       pre_done = true;
@@ -368,13 +427,8 @@ __device__ void finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
       if (t < cnt * NR) lds[t] = v[u];
     }
     __syncthreads();
-    // L lanes per row (a power of two up to a wavefront), reduced with cross-lane shuffles.
-    const int nrows = grp.y - grp.x;
-    int L = 64;
-    while (L > 1 && nrows * L > static_cast<int>(blockDim.x)) L >>= 1;
-    const int gi = threadIdx.x / L, lane = threadIdx.x & (L - 1), ng = blockDim.x / L;
-    for (int lr = grp.x + gi; lr < grp.y; lr += ng) {
-      const int s0 = P.lr_base[lr] - base0, ns = P.lr_nseg[lr];
+    if (mine) {
+      const int s0 = info.y - base0, ns = info.z;
       RowSum<NR> sum;
 #pragma unroll
       for (int r = 0; r < NR; ++r) sum.v[r] = 0.0;
@@ -386,7 +440,7 @@ __device__ void finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
 #pragma unroll
         for (int r = 0; r < NR; ++r) sum.v[r] += __shfl_xor(sum.v[r], off);
       }
-      if (lane == 0) epi(P.lr_row[lr], sum, acc);
+      if (lane == 0) epi(info.x, sum, ein, acc);
     }
     __syncthreads();
   }

@@ -34,7 +34,7 @@ class Result(C.Structure):
     _fields_ = [
         ("status", C.c_int32),
         ("iterations", C.c_int64), ("attempts", C.c_int64), ("restarts", C.c_int64),
-        ("ticks", C.c_int64), ("device_bytes", C.c_int64),
+        ("ticks", C.c_int64), ("device_bytes", C.c_int64), ("fp32_iterations", C.c_int64),
         ("objective", C.c_double), ("dual_bound", C.c_double), ("solve_seconds", C.c_double),
         ("setup_seconds", C.c_double), ("loop_seconds", C.c_double),
         ("spectral_norm", C.c_double),
@@ -108,6 +108,7 @@ def load_library() -> C.CDLL:
     lib.pdhg_dev_evaluate.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, _f64p]
     lib.pdhg_dev_start.argtypes = [vp, C.POINTER(DevParams)]
     lib.pdhg_dev_run.argtypes = [vp, C.c_int, C.POINTER(DevState)]
+    lib.pdhg_dev_promote.argtypes = [vp, C.POINTER(DevParams)]
     lib.pdhg_dev_ticks.argtypes = [vp, C.c_int64, C.POINTER(Tick), C.c_int64, C.POINTER(C.c_int64)]
     lib.pdhg_dev_finish.argtypes = [vp, _f64p, _f64p, _f64p, _f64p]
     lib.pdhg_dev_time_kernels.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -241,7 +242,7 @@ def solve(model: Model, **options) -> Solution:
                                 d.ctypes.data, act.ctypes.data, C.byref(res))
     if rc != 0:
         raise ValueError(res.message.decode())
-    stats = {k: getattr(res, k) for k in ("attempts", "restarts", "ticks", "device_bytes",
+    stats = {k: getattr(res, k) for k in ("attempts", "restarts", "ticks", "device_bytes", "fp32_iterations",
                                           "setup_seconds", "loop_seconds", "spectral_norm",
                                           "final_primal", "final_dual", "final_gap")}
     return Solution(STATUS_NAMES[res.status], res.objective, res.dual_bound, res.iterations,
@@ -314,6 +315,13 @@ class Device:
         p = DevParams(tolerance, iteration_limit, 1 if restart else 0, 1 if stop_at_request else 0,
                       spectral_norm, chunk, stop_rule)
         self._check(self._lib.pdhg_dev_start(self._h, C.byref(p)), "start")
+
+    def promote(self, tolerance=1e-8, iteration_limit=1000000, restart=True,
+                stop_at_request=False, spectral_norm=1.0, chunk=0, stop_rule=0):
+        """Mixed-precision handle: continue the same run in FP64 (pdhg_dev_promote)."""
+        p = DevParams(tolerance, iteration_limit, 1 if restart else 0, 1 if stop_at_request else 0,
+                      spectral_norm, chunk, stop_rule)
+        self._check(self._lib.pdhg_dev_promote(self._h, C.byref(p)), "promote")
 
     def run(self, chunks: int = 1) -> DevState:
         st = DevState()

@@ -165,6 +165,7 @@ int sankhya_pdhg_solve(const sankhya_model* h, const char* options, double* col_
     out->restarts = st.restarts;
     out->ticks = st.ticks;
     out->device_bytes = st.device_bytes;
+    out->fp32_iterations = st.fp32_iterations;
     out->objective = s.objective;
     out->dual_bound = s.dual_bound;
     out->solve_seconds = s.solve_seconds;

@@ -53,6 +53,10 @@ std::vector<OptionSpec> build_registry() {
   s.push_back({"pdhg_precision", OptionType::String, std::string("fp64"),
                "Bulk arithmetic: fp64, or mixed (FP32 matrix and iterates, FP64 reductions, step "
                "control and residuals).", 0, 0, {"fp64", "mixed"}});
+  s.push_back({"pdhg_mixed_switch", OptionType::Double, 1e-6,
+               "Mixed precision with pdhg_polish: the FP32 phase runs until the relative KKT error "
+               "is <= max(this, pdhg_tolerance), or stalls; the same run then continues in FP64 "
+               "to pdhg_tolerance.", 0.0, 1.0});
   s.push_back({"pdhg_stop_rule", OptionType::String, std::string("reference"),
                "reference: stop as SANKHYA's PDHG does (relative residuals <= pdhg_tolerance and "
                "absolute primal <= 1e-7, and the project standard unless pdhg_stop_at_request). "

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <limits>
 #include <random>
 #include <string>
 #include <vector>
@@ -201,6 +202,15 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   params.spectral_norm = spectral_norm;
   params.chunk_attempts = static_cast<int32_t>(options.get_int("pdhg_chunk"));
   params.stop_rule = options.get_string("pdhg_stop_rule") == "relative_kkt" ? 1 : 0;
+  // Mixed precision with the FP64 polish: the FP32 phase stops on the relative KKT error at
+  // max(pdhg_mixed_switch, tolerance); the same run then continues in FP64 under the requested
+  // rule and tolerance (pdhg_dev_promote). A stalled FP32 phase hands over early.
+  const pdhg_params final_params = params;
+  bool fp32_phase = mixed && options.get_bool("pdhg_polish");
+  if (fp32_phase) {
+    params.tolerance = std::max(tolerance, options.get_double("pdhg_mixed_switch"));
+    params.stop_rule = 1;
+  }
   rc = pdhg_dev_start(dev.h, &params);
   if (rc != PDHG_OK) return dev_error(rc, "start");
   const double loop_start = timer.elapsed_seconds();
@@ -211,6 +221,17 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   std::int64_t ticks_seen = 0;
   std::vector<pdhg_tick> tick_buf(4096);
   int batch = 1;
+  double stall_best = std::numeric_limits<double>::infinity();
+  std::int64_t stall_tick = 0;
+  constexpr std::int64_t kStallTicks = 25;  // FP32 phase: 25 ticks without a 10% gain
+  auto promote = [&]() {
+    g_last_stats.fp32_iterations = st.iteration;
+    logger.info("FP32 phase ended at iteration {} (relative KKT {:.3e}); continuing in FP64",
+                st.iteration, st.best_worst);
+    fp32_phase = false;
+    batch = 1;
+    return pdhg_dev_promote(dev.h, &final_params);
+  };
   while (true) {
     const double elapsed = timer.elapsed_seconds();
     if (has_time_limit && elapsed >= time_limit) {
@@ -256,6 +277,27 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
                          t.dual, timer.elapsed_seconds());
       }
       ticks_seen += got;
+    }
+    if (fp32_phase && st.done) {
+      // Already there when the request is a relative KKT error no tighter than the switch.
+      if (final_params.stop_rule == 1 && params.tolerance <= tolerance) {
+        g_last_stats.fp32_iterations = st.iteration;
+        converged = true;
+        break;
+      }
+      rc = promote();
+      if (rc != PDHG_OK) return dev_error(rc, "FP64 handoff");
+      continue;
+    }
+    if (fp32_phase && !st.limit_hit) {
+      if (st.best_worst < 0.9 * stall_best) {
+        stall_best = st.best_worst;
+        stall_tick = st.ticks;
+      } else if (st.ticks - stall_tick >= kStallTicks) {
+        rc = promote();
+        if (rc != PDHG_OK) return dev_error(rc, "FP64 handoff");
+        continue;
+      }
     }
     if (st.done) {
       converged = true;
