@@ -60,12 +60,21 @@ struct Ctl {
   int32_t done, limit_hit, use_restarts, stop_at_request, no_info, pw_zero, stop_rule;
 };
 
+// Iterate storage. FP64: xbuf/ybuf hold the iterates themselves. FP32 (mixed precision):
+// they hold deltas from the restart point, x = xrest + dx, y = yrest + dy, and axr = Ahat xrest,
+// atyr = Ahat^T yrest are kept in FP64 (recomputed after each restart). Rounding then scales
+// with the distance from the restart point, which shrinks as the run converges, instead of
+// with the iterate itself.
+template <typename X>
+constexpr bool kDelta = sizeof(X) == 4;
+
 template <typename X>
 struct Iter {
   X* xbuf;  // 2n: (x_a, x_b) per column
   X* ybuf;  // 2m
   double *xsum, *ysum, *xrest, *yrest, *bestx, *besty;
   const double *chat, *lhat, *uhat, *rlhat, *ruhat, *dc, *dr, *cost, *cl, *cu, *rl, *ru;
+  const double *axr, *atyr;  // delta form only
   int32_t n, m;
 };
 
@@ -103,7 +112,7 @@ struct PrimalEpi {
   int cur, pend;
   double tau, omega;
   struct In {
-    double x, xs, c, l, u;
+    double x, xs, c, l, u, xr, g0;
   };
   __device__ In load(int j) const {
     In r;
@@ -112,15 +121,32 @@ struct PrimalEpi {
     r.c = it.chat[j];
     r.l = it.lhat[j];
     r.u = it.uhat[j];
+    if constexpr (kDelta<X>) {
+      r.xr = it.xrest[j];
+      r.g0 = it.atyr[j];
+    } else {
+      r.xr = 0.0;
+      r.g0 = 0.0;
+    }
     return r;
   }
   __device__ void operator()(int j, const RowSum<1>& s, const In& in, Acc<1, 0>& a) const {
-    if (pend) it.xsum[j] = in.xs + in.x;
-    const double grad = in.c + s.v[0];
-    const X stored = static_cast<X>(project(in.x - tau * grad, in.l, in.u));
-    it.xbuf[2 * (int64_t)j + (cur ^ 1)] = stored;
-    const double d = static_cast<double>(stored) - in.x;
-    a.s[0] += 0.5 * omega * d * d;
+    if constexpr (kDelta<X>) {
+      const double x = in.xr + in.x;
+      if (pend) it.xsum[j] = in.xs + x;
+      const double grad = in.c + (in.g0 + s.v[0]);
+      const X stored = static_cast<X>(project(x - tau * grad, in.l, in.u) - in.xr);
+      it.xbuf[2 * (int64_t)j + (cur ^ 1)] = stored;
+      const double d = static_cast<double>(stored) - in.x;
+      a.s[0] += 0.5 * omega * d * d;
+    } else {
+      if (pend) it.xsum[j] = in.xs + in.x;
+      const double grad = in.c + s.v[0];
+      const X stored = static_cast<X>(project(in.x - tau * grad, in.l, in.u));
+      it.xbuf[2 * (int64_t)j + (cur ^ 1)] = stored;
+      const double d = static_cast<double>(stored) - in.x;
+      a.s[0] += 0.5 * omega * d * d;
+    }
   }
 };
 
@@ -145,7 +171,7 @@ struct DualEpi {
   int cur, pend;
   double sigma, omega;
   struct In {
-    double y, ys, rl, ru;
+    double y, ys, rl, ru, yr, g0;
   };
   __device__ In load(int i) const {
     In r;
@@ -153,16 +179,35 @@ struct DualEpi {
     r.ys = pend ? it.ysum[i] : 0.0;
     r.rl = it.rlhat[i];
     r.ru = it.ruhat[i];
+    if constexpr (kDelta<X>) {
+      r.yr = it.yrest[i];
+      r.g0 = it.axr[i];
+    } else {
+      r.yr = 0.0;
+      r.g0 = 0.0;
+    }
     return r;
   }
   __device__ void operator()(int i, const RowSum<2>& s, const In& in, Acc<2, 0>& a) const {
-    if (pend) it.ysum[i] = in.ys + in.y;
-    const double v = in.y + sigma * s.v[0];
-    const X stored = static_cast<X>(v - sigma * project(v / sigma, in.rl, in.ru));
-    it.ybuf[2 * (int64_t)i + (cur ^ 1)] = stored;
-    const double dy = static_cast<double>(stored) - in.y;
-    a.s[0] += 0.5 * dy * dy / omega;
-    a.s[1] += dy * s.v[1];
+    if constexpr (kDelta<X>) {
+      // ext = xrest + (2 dx' - dx), so Ahat ext = axr + s0; Ahat (x' - x) = s1 exactly.
+      const double y = in.yr + in.y;
+      if (pend) it.ysum[i] = in.ys + y;
+      const double v = y + sigma * (in.g0 + s.v[0]);
+      const X stored = static_cast<X>((v - sigma * project(v / sigma, in.rl, in.ru)) - in.yr);
+      it.ybuf[2 * (int64_t)i + (cur ^ 1)] = stored;
+      const double dy = static_cast<double>(stored) - in.y;
+      a.s[0] += 0.5 * dy * dy / omega;
+      a.s[1] += dy * s.v[1];
+    } else {
+      if (pend) it.ysum[i] = in.ys + in.y;
+      const double v = in.y + sigma * s.v[0];
+      const X stored = static_cast<X>(v - sigma * project(v / sigma, in.rl, in.ru));
+      it.ybuf[2 * (int64_t)i + (cur ^ 1)] = stored;
+      const double dy = static_cast<double>(stored) - in.y;
+      a.s[0] += 0.5 * dy * dy / omega;
+      a.s[1] += dy * s.v[1];
+    }
   }
 };
 
@@ -214,7 +259,8 @@ struct TickRowsGather {  // x_cur[j] and the running average x_avg[j], scaled
   int cur, tp;
   double count;
   __device__ RowSum<2> operator()(int j) const {
-    const double xc = static_cast<double>(it.xbuf[2 * (int64_t)j + cur]);
+    double xc = static_cast<double>(it.xbuf[2 * (int64_t)j + cur]);
+    if constexpr (kDelta<X>) xc += it.xrest[j];
     RowSum<2> g;
     g.v[0] = xc;
     g.v[1] = (tp ? it.xsum[j] + xc : it.xsum[j]) / count;
@@ -238,6 +284,7 @@ struct TickRowsEpi {
     r.rl = it.rl[i];
     r.ru = it.ru[i];
     r.yr = it.yrest[i];
+    if constexpr (kDelta<X>) r.yh += r.yr;
     return r;
   }
   __device__ void operator()(int i, const RowSum<2>& s, const In& in, Acc<7, 2>& a) const {
@@ -255,7 +302,8 @@ struct TickColsGather {
   int cur, tp;
   double count;
   __device__ RowSum<2> operator()(int i) const {
-    const double yc = static_cast<double>(it.ybuf[2 * (int64_t)i + cur]);
+    double yc = static_cast<double>(it.ybuf[2 * (int64_t)i + cur]);
+    if constexpr (kDelta<X>) yc += it.yrest[i];
     RowSum<2> g;
     g.v[0] = yc;
     g.v[1] = (tp ? it.ysum[i] + yc : it.ysum[i]) / count;
@@ -280,6 +328,7 @@ struct TickColsEpi {
     r.cl = it.cl[j];
     r.cu = it.cu[j];
     r.xr = it.xrest[j];
+    if constexpr (kDelta<X>) r.xh += r.xr;
     return r;
   }
   __device__ void operator()(int j, const RowSum<2>& s, const In& in, Acc<7, 2>& a) const {
@@ -706,23 +755,27 @@ __global__ __launch_bounds__(kThreads) void k_apply(Iter<X> it, Ctl* ctl) {
   const double count = static_cast<double>(ctl->eval_count > 0 ? ctl->eval_count : 1);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
   for (int64_t j = blockIdx.x * (int64_t)kThreads + threadIdx.x; j < it.n; j += stride) {
-    const double xc = static_cast<double>(it.xbuf[2 * j + cur]);
+    double xc = static_cast<double>(it.xbuf[2 * j + cur]);
+    if constexpr (kDelta<X>) xc += it.xrest[j];
     const double s = tp ? it.xsum[j] + xc : it.xsum[j];
     if (cb) it.bestx[j] = avg ? (s / count) * it.dc[j] : xc * it.dc[j];
     if (rs) {
       it.xsum[j] = 0.0;
       it.xrest[j] = xc;
+      if constexpr (kDelta<X>) it.xbuf[2 * j + cur] = static_cast<X>(0.0);
     } else {
       it.xsum[j] = s;
     }
   }
   for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < it.m; i += stride) {
-    const double yc = static_cast<double>(it.ybuf[2 * i + cur]);
+    double yc = static_cast<double>(it.ybuf[2 * i + cur]);
+    if constexpr (kDelta<X>) yc += it.yrest[i];
     const double s = tp ? it.ysum[i] + yc : it.ysum[i];
     if (cb) it.besty[i] = avg ? (s / count) * it.dr[i] : yc * it.dr[i];
     if (rs) {
       it.ysum[i] = 0.0;
       it.yrest[i] = yc;
+      if constexpr (kDelta<X>) it.ybuf[2 * i + cur] = static_cast<X>(0.0);
     } else {
       it.ysum[i] = s;
     }
@@ -730,13 +783,13 @@ __global__ __launch_bounds__(kThreads) void k_apply(Iter<X> it, Ctl* ctl) {
 }
 
 // Exact FP32 -> FP64 widening of the iterate pairs (the mixed-precision handoff).
-__global__ void k_widen(const float* xs, double* xd, int64_t nx, const float* ys, double* yd,
-                        int64_t ny) {
+__global__ void k_widen(const float* xs, const double* xr, double* xd, int64_t nx,
+                        const float* ys, const double* yr, double* yd, int64_t ny) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nx; k += stride)
-    xd[k] = static_cast<double>(xs[k]);
+    xd[k] = xr[k >> 1] + static_cast<double>(xs[k]);
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ny; k += stride)
-    yd[k] = static_cast<double>(ys[k]);
+    yd[k] = yr[k >> 1] + static_cast<double>(ys[k]);
 }
 
 __global__ void k_clear(Ctl* ctl) {
@@ -754,12 +807,12 @@ __global__ void k_init(Iter<X> it) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < it.n; j += stride) {
     const double x0 = project(0.0, it.lhat[j], it.uhat[j]);
-    it.xbuf[2 * j] = static_cast<X>(x0);
+    it.xbuf[2 * j] = static_cast<X>(kDelta<X> ? 0.0 : x0);
     it.xbuf[2 * j + 1] = static_cast<X>(0.0);
     it.xsum[j] = 0.0;
-    it.xrest[j] = static_cast<double>(static_cast<X>(x0));
+    it.xrest[j] = x0;
     // src/pdhg/pdhg.cpp:190: best_x starts as the scaled start point itself.
-    it.bestx[j] = static_cast<double>(static_cast<X>(x0));
+    it.bestx[j] = x0;
   }
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < it.m; i += stride) {
     it.ybuf[2 * i] = static_cast<X>(0.0);
@@ -880,6 +933,33 @@ __global__ __launch_bounds__(kTail) void k_spmv_tail(PlanView<V> P, double* y, c
   finish_long_rows<V, 1>(P, epi, acc, lds);
   block_reduce(acc, red);
   if (threadIdx.x == 0) lr_sum[0] = acc.s[0];
+}
+
+// Anchor products of the mixed-precision delta form: y = P x in FP64 (x = a restart point), run
+// after a restart has moved the restart point (ctl->restart_now) or when forced at start.
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_anchor(PlanView<V> P, const double* x, double* y,
+                                                     const Ctl* ctl, int force) {
+  __shared__ double lds[kTile];
+  __shared__ double red[4];
+  const int4 item = P.items[blockIdx.x];
+  if (!force && !ctl->restart_now) return;
+  PlainGather gather{x, 0.0, 0.0};
+  PlainEpi epi{y};
+  Acc<1, 0> acc;
+  acc.zero();
+  run_item<V, 1>(P, item, gather, epi, acc, lds, red);
+}
+
+template <typename V>
+__global__ __launch_bounds__(kTail) void k_anchor_tail(PlanView<V> P, double* y, const Ctl* ctl,
+                                                       int force) {
+  __shared__ double lds[kChunk];
+  if (!force && !ctl->restart_now) return;
+  PlainEpi epi{y};
+  Acc<1, 0> acc;
+  acc.zero();
+  finish_long_rows<V, 1>(P, epi, acc, lds);
 }
 
 // Power-iteration step end (src/la/scaling.cpp:185-187): next = ||v||^2, norm = next^(1/4);
@@ -1042,6 +1122,7 @@ struct pdhg_dev {
   double *xb64 = nullptr, *yb64 = nullptr;   // iterate pairs, FP64
   double *xsum = nullptr, *ysum = nullptr, *xrest = nullptr, *yrest = nullptr;
   double *bestx = nullptr, *besty = nullptr, *sx = nullptr, *sy = nullptr, *sx2 = nullptr;
+  double *axr = nullptr, *atyr = nullptr;  // Ahat xrest, Ahat^T yrest (mixed-precision delta form)
   double* sact = nullptr;
   double *chat = nullptr, *lhat = nullptr, *uhat = nullptr, *rlhat = nullptr, *ruhat = nullptr;
   double *dc = nullptr, *dr = nullptr, *cost = nullptr, *cl = nullptr, *cu = nullptr,
@@ -1222,6 +1303,8 @@ Iter<X> iter_of(const pdhg_dev* h) {
   it.cu = h->cu;
   it.rl = h->rl;
   it.ru = h->ru;
+  it.axr = h->axr;
+  it.atyr = h->atyr;
   it.n = h->n;
   it.m = h->m;
   return it;
@@ -1248,6 +1331,20 @@ void launch_attempt(pdhg_dev* h, int force) {
                                                 h->A.nitems, force);
 }
 
+// Anchor products Ahat xrest and Ahat^T yrest in FP64 (delta form), gated on a restart.
+void launch_anchor(pdhg_dev* h, int force) {
+  if (h->A.nitems > 0)
+    k_anchor<double><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<double>(), h->xrest, h->axr,
+                                                              h->ctl, force);
+  if (h->A.nlr > 0)
+    k_anchor_tail<double><<<1, kTail, 0, h->stream>>>(h->A.view<double>(), h->axr, h->ctl, force);
+  if (h->T.nitems > 0)
+    k_anchor<double><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<double>(), h->yrest,
+                                                              h->atyr, h->ctl, force);
+  if (h->T.nlr > 0)
+    k_anchor_tail<double><<<1, kTail, 0, h->stream>>>(h->T.view<double>(), h->atyr, h->ctl, force);
+}
+
 template <typename V, typename X>
 void launch_chunk(pdhg_dev* h) {
   const Iter<X> it = iter_of<X>(h);
@@ -1264,6 +1361,7 @@ void launch_chunk(pdhg_dev* h) {
                                                   h->ctl, h->part_er, h->A.nitems, h->part_ec,
                                                   h->T.nitems, h->ticks);
   k_apply<X><<<apply_grid(h), kThreads, 0, h->stream>>>(it, h->ctl);
+  if constexpr (kDelta<X>) launch_anchor(h, 0);
   k_clear<<<1, 64, 0, h->stream>>>(h->ctl);
 }
 
@@ -1425,10 +1523,10 @@ int pdhg_dev_upload(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz, const int32_
     return rc;
   }
   const size_t un = static_cast<size_t>(n), um = static_cast<size_t>(m);
-  double** nvec[] = {&h->xsum, &h->xrest, &h->bestx, &h->sx, &h->sx2};
+  double** nvec[] = {&h->xsum, &h->xrest, &h->bestx, &h->sx, &h->sx2, &h->atyr};
   for (double** v : nvec)
     if ((rc = dalloc(h, v, un))) return rc;
-  double** mvec[] = {&h->ysum, &h->yrest, &h->besty, &h->sy, &h->sact};
+  double** mvec[] = {&h->ysum, &h->yrest, &h->besty, &h->sy, &h->sact, &h->axr};
   for (double** v : mvec)
     if ((rc = dalloc(h, v, um))) return rc;
   if ((rc = upload(h, &h->chat, cost_hat, un)) || (rc = upload(h, &h->lhat, lower_hat, un)) ||
@@ -1546,8 +1644,12 @@ int pdhg_dev_start(pdhg_dev* h, const pdhg_params* p) {
   }
   HIP_TRY(h, hipMemcpyAsync(h->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
   const int grid = apply_grid(h);
-  if (mixed(h)) k_init<float><<<grid, kThreads, 0, h->stream>>>(iter_of<float>(h));
-  else k_init<double><<<grid, kThreads, 0, h->stream>>>(iter_of<double>(h));
+  if (mixed(h)) {
+    k_init<float><<<grid, kThreads, 0, h->stream>>>(iter_of<float>(h));
+    launch_anchor(h, 1);
+  } else {
+    k_init<double><<<grid, kThreads, 0, h->stream>>>(iter_of<double>(h));
+  }
   HIP_TRY(h, hipGetLastError());
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   const int chunk = p->chunk_attempts > 0 ? p->chunk_attempts : kDefaultChunk;
@@ -1573,8 +1675,8 @@ int pdhg_dev_promote(pdhg_dev* h, const pdhg_params* p) {
   }
   const int64_t w = std::max<int64_t>(std::max<int64_t>(2 * int64_t{h->n}, 2 * int64_t{h->m}), 1);
   const int grid = static_cast<int>(std::min<int64_t>((w + kThreads - 1) / kThreads, 8192));
-  k_widen<<<grid, kThreads, 0, h->stream>>>(h->xb32, h->xb64, 2 * int64_t{h->n}, h->yb32, h->yb64,
-                                            2 * int64_t{h->m});
+  k_widen<<<grid, kThreads, 0, h->stream>>>(h->xb32, h->xrest, h->xb64, 2 * int64_t{h->n},
+                                            h->yb32, h->yrest, h->yb64, 2 * int64_t{h->m});
   HIP_TRY(h, hipGetLastError());
   Ctl c;
   HIP_TRY(h, hipMemcpyAsync(&c, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));

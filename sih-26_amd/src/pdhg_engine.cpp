@@ -223,7 +223,9 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   int batch = 1;
   double stall_best = std::numeric_limits<double>::infinity();
   std::int64_t stall_tick = 0;
-  constexpr std::int64_t kStallTicks = 25;  // FP32 phase: 25 ticks without a 10% gain
+  // FP32 phase stall: no 10% gain in the best KKT error for max(100, ticks so far / 2) ticks
+  // (progress comes in bursts at restarts, whose spacing grows with the iteration count).
+  constexpr std::int64_t kStallTicks = 100;
   auto promote = [&]() {
     g_last_stats.fp32_iterations = st.iteration;
     logger.info("FP32 phase ended at iteration {} (relative KKT {:.3e}); continuing in FP64",
@@ -293,7 +295,7 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
       if (st.best_worst < 0.9 * stall_best) {
         stall_best = st.best_worst;
         stall_tick = st.ticks;
-      } else if (st.ticks - stall_tick >= kStallTicks) {
+      } else if (st.ticks - stall_tick >= std::max(kStallTicks, st.ticks / 2)) {
         rc = promote();
         if (rc != PDHG_OK) return dev_error(rc, "FP64 handoff");
         continue;
