@@ -61,6 +61,15 @@ int sankhya_pdhg_solve(const sankhya_model* model, const char* options, double* 
                        double* row_dual, double* col_dual, double* row_activity,
                        sankhya_result* result);
 
+/* Solve nlp independent LPs with the batched engine (include/pdhg_batch.h); same options as
+ * sankhya_pdhg_solve plus pdhg_batch_tune / pdhg_batch_tune_attempts. Outputs concatenated in
+ * model order: col_value and col_dual hold sum(n) values, row_dual and row_activity sum(m);
+ * results holds nlp records. Any output pointer may be NULL. Returns 0, or -1 when an option is
+ * rejected (results[0].message says which). */
+int sankhya_pdhg_solve_batch(const sankhya_model* const* models, int32_t nlp, const char* options,
+                             double* col_value, double* row_dual, double* col_dual,
+                             double* row_activity, sankhya_result* results);
+
 /* The host preconditioner on its own (src/la/scaling.cpp:111-159): Dr (m), Dc (n), the scaled
  * CSC values (nnz), scaled cost/bounds. min_cost is sense * c. */
 int sankhya_pdhg_scaling(const sankhya_model* model, int ruiz_iterations, double* row,

@@ -227,9 +227,18 @@ struct RowSum {
   double v[NR];
 };
 
-// Rows one lane finishes in a stream item, at most: a lane group of L lanes owns rows
-// grp, grp + ngrp, ...; with L > 1 an item has at most ngrp rows, with L = 1 at most kMaxRows.
-constexpr int kRowsPerLane = kMaxRows / kThreads;
+// Work-item geometry of one kernel family: threads per workgroup, nonzeros per stream item and
+// per long-row segment, rows per stream item. kRowsPerLane is how many rows one lane finishes
+// in a stream item at most: a lane group of L lanes owns rows grp, grp + ngrp, ...; with L > 1
+// an item has at most ngrp rows, with L = 1 at most kMaxRows.
+template <int NT, int TILE, int SEG, int MAXROWS>
+struct Geometry {
+  static constexpr int kThreads = NT, kTile = TILE, kSeg = SEG, kMaxRows = MAXROWS;
+  static constexpr int kRowsPerLane = MAXROWS / NT;
+  static_assert(MAXROWS % NT == 0 && TILE % (2 * NT) == 0, "geometry");
+};
+// The single-LP engine: 256-thread workgroups, one item each.
+using SingleGeo = Geometry<kThreads, kTile, kSeg, kMaxRows>;
 
 // Process one item. gather(j) returns RowSum<NR> for column j. The epilogue type E provides
 // E::In load(row) (the row's own inputs: iterate, bounds, running sum) and
@@ -239,9 +248,11 @@ constexpr int kRowsPerLane = kMaxRows / kThreads;
 // Memory round trips: the item descriptor arrives with the caller's control-block loads; then
 // ONE round trip brings the item's nonzeros, its row extents and every epilogue input; then the
 // gathers. Products are staged in LDS and each row is summed by L lanes with shuffles.
-template <typename V, int NR, int NS, int NM, class G, class E>
-__device__ void run_item(const PlanView<V>& P, const int4 it, G& gather, E& epi,
-                         Acc<NS, NM>& acc, double* lds, double* red) {
+template <class GEO, typename V, int NR, int NS, int NM, class G, class E>
+__device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& epi,
+                             Acc<NS, NM>& acc, double* lds, double* red) {
+  constexpr int kThreads = GEO::kThreads, kTile = GEO::kTile, kSeg = GEO::kSeg;
+  constexpr int kRowsPerLane = GEO::kRowsPerLane;
   const int tid = threadIdx.x;
   if (it.y >= 0) {
     const int r0 = it.x, r1 = it.y, k0 = it.z, k1 = it.w;
@@ -383,6 +394,12 @@ __device__ void run_item(const PlanView<V>& P, const int4 it, G& gather, E& epi,
   }
 }
 
+template <typename V, int NR, int NS, int NM, class G, class E>
+__device__ __forceinline__ void run_item(const PlanView<V>& P, const int4 it, G& gather, E& epi,
+                                         Acc<NS, NM>& acc, double* lds, double* red) {
+  run_item_geo<SingleGeo, V, NR>(P, it, gather, epi, acc, lds, red);
+}
+
 // Completes every long row of P in one workgroup (launched after the kernel that wrote the
 // segment sums): for each group of at most blockDim long rows, stage their segment sums in
 // LDS with coalesced loads while each row's lane fetches the row's epilogue inputs; then a
@@ -393,9 +410,9 @@ struct NoPre {
   __device__ void operator()() const {}
 };
 
-template <typename V, int NR, int NS, int NM, class E, class Pre = NoPre>
-__device__ void finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
-                                 double* lds /* kChunk * NR */, Pre pre = Pre()) {
+template <int NTAIL, typename V, int NR, int NS, int NM, class E, class Pre = NoPre>
+__device__ void finish_long_rows_geo(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
+                                     double* lds /* kChunk * NR */, Pre pre = Pre()) {
   bool pre_done = false;
   for (int g = 0; g < P.ngroups; ++g) {
     const int4 grp = P.lr_groups[g];  // {first, last, base0, segment count}
@@ -407,8 +424,8 @@ __device__ void finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
     const bool mine = gi < nrows;
     int4 info = make_int4(0, 0, 0, 0);
     if (mine) info = P.lr_info[lr];
-    // kChunk * NR / kTail loads per lane at most, held in registers until `pre` has run.
-    constexpr int U = (kChunk * NR + kTail - 1) / kTail;
+    // kChunk * NR / NTAIL loads per lane at most, held in registers until `pre` has run.
+    constexpr int U = (kChunk * NR + NTAIL - 1) / NTAIL;
     double v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -445,6 +462,13 @@ __device__ void finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
     __syncthreads();
   }
   if (!pre_done) pre();
+}
+
+// The single-LP engine's tail kernels run kTail threads.
+template <typename V, int NR, int NS, int NM, class E, class Pre = NoPre>
+__device__ __forceinline__ void finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
+                                                 double* lds, Pre pre = Pre()) {
+  finish_long_rows_geo<kTail, V, NR>(P, epi, acc, lds, pre);
 }
 
 // Adds p[q * stride + k] for q in [0, count) into out[k], k < NV, with the loads of several

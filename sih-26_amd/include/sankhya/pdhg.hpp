@@ -6,6 +6,8 @@
 // that says so.
 #pragma once
 
+#include <vector>
+
 #include "sankhya/logging.hpp"
 #include "sankhya/model.hpp"
 #include "sankhya/options.hpp"
@@ -24,6 +26,15 @@ namespace sankhya::gpu {
 /// (src/gpu/pdhg_gpu.hpp:19-20).
 [[nodiscard]] Solution solve_pdhg_gpu(const Model& model, const Options& options,
                                       Logger& logger, SolveControl* control = nullptr);
+
+/// Many independent LPs solved together by the batched engine (no reference counterpart: the
+/// reference solves one LP per call). Each LP runs the same algorithm as solve_pdhg_gpu under
+/// the same options; the batch adds a schedule (one workgroup per LP in a persistent launch)
+/// and, with pdhg_batch_tune, per-LP workgroup geometries picked by timing. Solutions come
+/// back in the order of `models`. Never throws.
+[[nodiscard]] std::vector<Solution> solve_pdhg_batch(const std::vector<const Model*>& models,
+                                                     const Options& options, Logger& logger,
+                                                     SolveControl* control = nullptr);
 
 /// Is a gfx950 device present? Fills `description` (reference src/gpu/device.hpp).
 [[nodiscard]] bool device_available(std::string* description);

@@ -96,6 +96,9 @@ def load_library() -> C.CDLL:
     lib.sankhya_model_free.argtypes = [vp]
     lib.sankhya_pdhg_solve.restype = C.c_int
     lib.sankhya_pdhg_solve.argtypes = [vp, C.c_char_p, vp, vp, vp, vp, C.POINTER(Result)]
+    lib.sankhya_pdhg_solve_batch.restype = C.c_int
+    lib.sankhya_pdhg_solve_batch.argtypes = [C.POINTER(vp), C.c_int32, C.c_char_p, vp, vp, vp, vp,
+                                             C.POINTER(Result)]
     lib.sankhya_pdhg_scaling.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
                                          _f64p, _f64p]
     lib.sankhya_pdhg_device_open.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
@@ -249,6 +252,42 @@ def solve(model: Model, **options) -> Solution:
                     x[:n], y[:m], d[:n], act[:m], res.algorithm.decode(), res.message.decode(),
                     res.solve_seconds, res.primal_infeasibility, res.dual_infeasibility,
                     res.complementarity_violation, res.relative_gap, stats)
+
+
+def _solution_of(res, x, y, d, act, stats_extra=None) -> Solution:
+    stats = {k: getattr(res, k) for k in ("attempts", "restarts", "ticks", "device_bytes", "fp32_iterations",
+                                          "setup_seconds", "loop_seconds", "spectral_norm",
+                                          "final_primal", "final_dual", "final_gap")}
+    return Solution(STATUS_NAMES[res.status], res.objective, res.dual_bound, res.iterations,
+                    x, y, d, act, res.algorithm.decode(), res.message.decode(),
+                    res.solve_seconds, res.primal_infeasibility, res.dual_infeasibility,
+                    res.complementarity_violation, res.relative_gap, stats)
+
+
+def solve_batch(models, **options):
+    """Solve independent LPs together with the batched engine (sankhya_pdhg_solve_batch).
+    Options as for solve(), plus pdhg_batch_tune and pdhg_batch_tune_attempts. Returns one
+    Solution per model, in order."""
+    lib = load_library()
+    models = list(models)
+    nlp = len(models)
+    dims = [md.dims for md in models]
+    sn = sum(d[1] for d in dims)
+    sm = sum(d[0] for d in dims)
+    x, d, y, act = np.zeros(max(sn, 1)), np.zeros(max(sn, 1)), np.zeros(max(sm, 1)), np.zeros(max(sm, 1))
+    res = (Result * max(nlp, 1))()
+    handles = (C.c_void_p * max(nlp, 1))(*[md._h.value for md in models])
+    rc = lib.sankhya_pdhg_solve_batch(handles, nlp, _options_text(options), x.ctypes.data, y.ctypes.data,
+                                      d.ctypes.data, act.ctypes.data, res)
+    if rc != 0:
+        raise ValueError(res[0].message.decode())
+    out, on, om = [], 0, 0
+    for k, (m, n, _) in enumerate(dims):
+        out.append(_solution_of(res[k], x[on:on + n].copy(), y[om:om + m].copy(), d[on:on + n].copy(),
+                                act[om:om + m].copy()))
+        on += n
+        om += m
+    return out
 
 
 def host_scaling(model: Model, ruiz: int = 10) -> Dict[str, np.ndarray]:

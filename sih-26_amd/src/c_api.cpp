@@ -185,6 +185,70 @@ int sankhya_pdhg_solve(const sankhya_model* h, const char* options, double* col_
   return 0;
 }
 
+int sankhya_pdhg_solve_batch(const sankhya_model* const* models, int32_t nlp, const char* options,
+                             double* col_value, double* row_dual, double* col_dual,
+                             double* row_activity, sankhya_result* results) {
+  Options o;
+  o.set_bool("log_to_console", false);
+  std::string error;
+  if (!apply_options(&o, options, &error) || nlp < 0 || (nlp > 0 && !models)) {
+    if (results && nlp > 0) {
+      std::memset(results, 0, sizeof(*results));
+      results->status = static_cast<int32_t>(SolveStatus::kModelError);
+      copy_text(results->message, sizeof(results->message), error.empty() ? "bad batch" : error);
+    }
+    return -1;
+  }
+  std::vector<const Model*> list(static_cast<std::size_t>(nlp));
+  for (int32_t k = 0; k < nlp; ++k) list[k] = models[k] ? &models[k]->model : nullptr;
+  Logger logger(o.get_bool("log_to_console") ? stdout : nullptr, LogLevel::kInfo);
+  const std::vector<Solution> sols = gpu::solve_pdhg_batch(list, o, logger, nullptr);
+  const auto& stats = hip_engine::last_batch_stats();
+  std::size_t on = 0, om = 0;
+  for (int32_t k = 0; k < nlp; ++k) {
+    const Solution& s = sols[k];
+    const std::size_t n = models[k] ? static_cast<std::size_t>(models[k]->model.num_cols()) : 0;
+    const std::size_t m = models[k] ? static_cast<std::size_t>(models[k]->model.num_rows()) : 0;
+    auto cp = [](double* dst, std::size_t off, const std::vector<double>& src, std::size_t len) {
+      if (dst && src.size() == len && len) std::memcpy(dst + off, src.data(), sizeof(double) * len);
+    };
+    cp(col_value, on, s.col_value, n);
+    cp(col_dual, on, s.col_dual, n);
+    cp(row_dual, om, s.row_dual, m);
+    cp(row_activity, om, s.row_activity, m);
+    on += n;
+    om += m;
+    if (!results) continue;
+    sankhya_result* out = results + k;
+    std::memset(out, 0, sizeof(*out));
+    out->status = static_cast<int32_t>(s.status);
+    out->iterations = s.iterations;
+    if (static_cast<std::size_t>(k) < stats.size()) {
+      const auto& st = stats[k];
+      out->attempts = st.attempts;
+      out->restarts = st.restarts;
+      out->ticks = st.ticks;
+      out->device_bytes = st.device_bytes;
+      out->setup_seconds = st.setup_seconds;
+      out->loop_seconds = st.loop_seconds;
+      out->spectral_norm = st.spectral_norm;
+      out->final_primal = st.final_primal;
+      out->final_dual = st.final_dual;
+      out->final_gap = st.final_gap;
+    }
+    out->objective = s.objective;
+    out->dual_bound = s.dual_bound;
+    out->solve_seconds = s.solve_seconds;
+    out->primal_infeasibility = s.primal_infeasibility;
+    out->dual_infeasibility = s.dual_infeasibility;
+    out->complementarity_violation = s.complementarity_violation;
+    out->relative_gap = s.relative_gap;
+    copy_text(out->algorithm, sizeof(out->algorithm), s.algorithm);
+    copy_text(out->message, sizeof(out->message), s.message);
+  }
+  return 0;
+}
+
 int sankhya_pdhg_scaling(const sankhya_model* h, int ruiz, double* row, double* column,
                          double* scaled_val, double* cost_hat, double* col_lower_hat,
                          double* col_upper_hat, double* row_lower_hat, double* row_upper_hat) {

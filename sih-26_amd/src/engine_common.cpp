@@ -1,0 +1,208 @@
+// SPDX-License-Identifier: Apache-2.0
+// Host steps shared by the single-LP and the batched engine (see engine_common.hpp). Host code
+// only: nothing here touches the device.
+#include "engine_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <random>
+#include <atomic>
+#include <string>
+#include <thread>
+
+namespace sankhya::hip_engine {
+
+Residuals from_array(const double* r) {
+  Residuals x;
+  x.primal = r[0];
+  x.dual = r[1];
+  x.gap = r[2];
+  x.primal_objective = r[3];
+  x.dual_objective = r[4];
+  x.absolute_primal = r[5];
+  x.absolute_dual = r[6];
+  x.gap_as_verified = r[7];
+  x.complementarity = r[8];
+  return x;
+}
+
+// CSR of a CSC pattern by a counting-sort transpose: column indices ascend within each row
+// (reference CsrView::build, src/la/sparse.cpp:271-322).
+void csc_to_csr(Index m, Index n, const std::vector<Index>& cp, const std::vector<Index>& ri,
+                const std::vector<double>& v, std::vector<Index>* rp, std::vector<Index>* ci,
+                std::vector<double>* cv) {
+  const std::size_t nnz = v.size();
+  rp->assign(static_cast<std::size_t>(m) + 1, 0);
+  for (std::size_t k = 0; k < nnz; ++k) ++(*rp)[static_cast<std::size_t>(ri[k]) + 1];
+  for (Index i = 0; i < m; ++i) (*rp)[i + 1] += (*rp)[i];
+  ci->resize(nnz);
+  cv->resize(nnz);
+  std::vector<Index> next(rp->begin(), rp->end() - 1);
+  for (Index j = 0; j < n; ++j) {
+    for (Index k = cp[j]; k < cp[j + 1]; ++k) {
+      const auto slot = static_cast<std::size_t>(next[ri[k]]++);
+      (*ci)[slot] = j;
+      (*cv)[slot] = v[k];
+    }
+  }
+}
+
+Prepared prepare(const Model& model) {
+  Prepared p;
+  const Index rows = model.num_rows(), cols = model.num_cols();
+  const double sense = model.sense_multiplier();
+  const auto n = static_cast<std::size_t>(cols), m = static_cast<std::size_t>(rows);
+  // The unscaled problem, in minimise space (pdhg.cpp:94-111).
+  p.cost.resize(n);
+  for (std::size_t j = 0; j < n; ++j) p.cost[j] = sense * model.col_cost[j];
+  double cost_square = 0.0;
+  for (const double c : p.cost) cost_square += c * c;
+  p.cost_norm = std::sqrt(cost_square);
+  double bound_square = 0.0;
+  for (std::size_t i = 0; i < m; ++i) {
+    const double b = is_finite_bound(model.row_lower[i])
+                         ? model.row_lower[i]
+                         : (is_finite_bound(model.row_upper[i]) ? model.row_upper[i] : 0.0);
+    bound_square += b * b;
+  }
+  p.bound_norm = std::sqrt(bound_square);
+  // Preconditioning (pdhg.cpp:113-117) and the CSR of Ahat the device reads next to the CSC.
+  p.scaling = build_scaling(model, p.cost, kRuizIterations);
+  csc_to_csr(rows, cols, model.matrix.column_starts(), model.matrix.row_indices(),
+             p.scaling.values, &p.a_rp, &p.a_ci, &p.a_v);
+  return p;
+}
+
+std::vector<double> power_start(const Model& model, std::int64_t random_seed) {
+  // std::mt19937(random_seed + 1), U(-1, 1) (src/la/scaling.cpp:168-171).
+  std::vector<double> v0(static_cast<std::size_t>(model.num_cols()));
+  std::mt19937 rng(static_cast<unsigned>(random_seed) + 1u);
+  std::uniform_real_distribution<double> spread(-1.0, 1.0);
+  for (double& v : v0) v = spread(rng);
+  return v0;
+}
+
+void report(const Model& model, const double* res9, const std::vector<double>& y,
+            const std::vector<double>& reduced, const ReportState& rs, Solution* solution) {
+  const Residuals final_r = from_array(res9);
+  const double sense = model.sense_multiplier();
+  const std::size_t n = reduced.size(), m = y.size();
+  for (std::size_t j = 0; j < n; ++j) solution->col_dual[j] = sense * reduced[j];
+  for (std::size_t i = 0; i < m; ++i) solution->row_dual[i] = sense * (-y[i]);
+  solution->iterations = rs.iterations;
+  const bool verifiable = rs.converged && final_r.meets_standard();
+  if (verifiable) {
+    solution->status = SolveStatus::kOptimal;
+    solution->message = detail::format(
+        "MI355X PDHG converged after {} iterations and {} restarts; absolute primal {:.3e}, dual "
+        "{:.3e}, relative gap {:.3e}",
+        rs.iterations, rs.restarts, final_r.absolute_primal, final_r.absolute_dual,
+        final_r.gap_as_verified);
+  } else if (rs.converged) {
+    solution->status = SolveStatus::kFeasible;
+    solution->message = detail::format(
+        "met the requested relative tolerance {:.1e} after {} iterations, but NOT the absolute "
+        "standard this project verifies against (primal {:.3e} vs {:.1e}, dual {:.3e} vs {:.1e}, "
+        "relative gap {:.3e} vs {:.1e}). Reported as feasible, not optimal. {}",
+        rs.tolerance, rs.iterations, final_r.absolute_primal, tol::kPrimalFeasibility,
+        final_r.absolute_dual, tol::kDualFeasibility, final_r.gap_as_verified, tol::kDualityGap,
+        rs.stop_at_request ? "pdhg_stop_at_request is on, so this is the cheap answer that was "
+                             "asked for; turn it off to run on to the standard"
+                           : "Tighten --option pdhg_tolerance to close it");
+  } else {
+    solution->status =
+        (rs.stop_status == SolveStatus::kTimeLimit || rs.stop_status == SolveStatus::kInterrupted)
+            ? rs.stop_status
+            : SolveStatus::kIterationLimit;
+    solution->message = detail::format(
+        "stopped at relative primal {:.3e}, dual {:.3e}, gap {:.3e} after {} iterations and {} "
+        "restarts (target {:.1e})",
+        final_r.primal, final_r.dual, final_r.gap, rs.iterations, rs.restarts, rs.tolerance);
+  }
+  solution->stopped_by = solution->status == SolveStatus::kTimeLimit     ? LimitReason::kTime
+                         : solution->status == SolveStatus::kInterrupted ? LimitReason::kInterrupt
+                         : solution->status == SolveStatus::kIterationLimit
+                             ? LimitReason::kIterations
+                             : LimitReason::kNone;
+  solution->dual_bound = verifiable ? sense * final_r.dual_objective + model.objective_offset
+                                    : (model.sense == ObjSense::kMaximize ? kInfinity : -kInfinity);
+  solution->recompute_quality(model);
+}
+
+namespace {
+
+int device_from_option(const std::string& text, Logger& logger) {
+  if (text.empty() || text == "auto") return 0;
+  const std::size_t comma = text.find(',');
+  if (comma != std::string::npos)
+    logger.warning("gpu_devices lists several devices; this engine runs on one GPU and uses {}",
+                   text.substr(0, comma));
+  return std::atoi(text.substr(0, comma).c_str());
+}
+
+}  // namespace
+
+int engine_device(const Options& options, Logger& logger) {
+  return device_from_option(options.get_string("gpu_devices"), logger);
+}
+
+void prepare_all(const std::vector<const Model*>& models, std::int64_t random_seed,
+                 std::vector<Prepared>* prep, std::vector<std::vector<double>>* v0) {
+  const std::size_t count = models.size();
+  prep->assign(count, Prepared{});
+  v0->assign(count, std::vector<double>{});
+  const std::size_t workers =
+      std::max<std::size_t>(1, std::min<std::size_t>(count, std::thread::hardware_concurrency()));
+  std::atomic<std::size_t> next{0};
+  auto work = [&] {
+    for (std::size_t k; (k = next.fetch_add(1)) < count;) {
+      (*prep)[k] = prepare(*models[k]);
+      (*v0)[k] = power_start(*models[k], random_seed);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (std::size_t w = 0; w + 1 < workers; ++w) pool.emplace_back(work);
+  work();
+  for (std::thread& t : pool) t.join();
+}
+
+BatchInput make_batch_input(const std::vector<const Model*>& models,
+                            const std::vector<Prepared>& prep,
+                            const std::vector<std::vector<double>>& v0) {
+  BatchInput b;
+  for (std::size_t q = 0; q < models.size(); ++q) {
+    const Model& md = *models[q];
+    const Prepared& p = prep[q];
+    b.m.push_back(md.num_rows());
+    b.n.push_back(md.num_cols());
+    b.nnz.push_back(md.num_nonzeros());
+    b.a_rp.insert(b.a_rp.end(), p.a_rp.begin(), p.a_rp.end());
+    b.a_ci.insert(b.a_ci.end(), p.a_ci.begin(), p.a_ci.end());
+    b.a_v.insert(b.a_v.end(), p.a_v.begin(), p.a_v.end());
+    b.t_rp.insert(b.t_rp.end(), md.matrix.column_starts().begin(), md.matrix.column_starts().end());
+    b.t_ci.insert(b.t_ci.end(), md.matrix.row_indices().begin(), md.matrix.row_indices().end());
+    b.t_v.insert(b.t_v.end(), p.scaling.values.begin(), p.scaling.values.end());
+    auto add = [](std::vector<double>& dst, const std::vector<double>& src) {
+      dst.insert(dst.end(), src.begin(), src.end());
+    };
+    add(b.chat, p.scaling.cost);
+    add(b.lhat, p.scaling.col_lower);
+    add(b.uhat, p.scaling.col_upper);
+    add(b.rlhat, p.scaling.row_lower);
+    add(b.ruhat, p.scaling.row_upper);
+    add(b.dc, p.scaling.column);
+    add(b.dr, p.scaling.row);
+    add(b.cost, p.cost);
+    add(b.cl, md.col_lower);
+    add(b.cu, md.col_upper);
+    add(b.rl, md.row_lower);
+    add(b.ru, md.row_upper);
+    add(b.v0, v0[q]);
+    b.bound_norm.push_back(p.bound_norm);
+    b.cost_norm.push_back(p.cost_norm);
+  }
+  return b;
+}
+
+}  // namespace sankhya::hip_engine

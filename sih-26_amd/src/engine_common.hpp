@@ -1,0 +1,79 @@
+// SPDX-License-Identifier: Apache-2.0
+// Host steps shared by the single-LP and the batched engine: the problem's norms and
+// preconditioning with the CSR the device reads (src/pdhg/pdhg.cpp:94-117), the power
+// iteration's start vector (src/la/scaling.cpp:168-171), and the report that turns the device's
+// final residuals into a Solution (src/pdhg/pdhg.cpp:486-567).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "sankhya/model.hpp"
+#include "sankhya/options.hpp"
+#include "sankhya/logging.hpp"
+#include "scaling.hpp"
+
+namespace sankhya::hip_engine {
+
+constexpr int kRuizIterations = 10;   // src/pdhg/pdhg.cpp:62
+constexpr int kPowerIterations = 30;  // src/pdhg/pdhg.cpp:63
+
+struct Residuals {
+  double primal = 0, dual = 0, gap = 0, primal_objective = 0, dual_objective = 0;
+  double absolute_primal = 0, absolute_dual = 0, gap_as_verified = 0, complementarity = 0;
+  [[nodiscard]] bool meets_standard() const {
+    return absolute_primal <= tol::kPrimalFeasibility && absolute_dual <= tol::kDualFeasibility &&
+           gap_as_verified <= tol::kDualityGap && complementarity <= 1e-6;
+  }
+};
+Residuals from_array(const double* r);
+
+void csc_to_csr(Index m, Index n, const std::vector<Index>& cp, const std::vector<Index>& ri,
+                const std::vector<double>& v, std::vector<Index>* rp, std::vector<Index>* ci,
+                std::vector<double>* cv);
+
+// One LP as the device receives it.
+struct Prepared {
+  std::vector<double> cost;  // minimise-space cost
+  double cost_norm = 0.0, bound_norm = 0.0;
+  Scaling scaling;
+  std::vector<Index> a_rp, a_ci;  // CSR of Ahat
+  std::vector<double> a_v;
+};
+Prepared prepare(const Model& model);
+
+std::vector<double> power_start(const Model& model, std::int64_t random_seed);
+
+struct ReportState {
+  bool converged;
+  SolveStatus stop_status;
+  std::int64_t iterations, restarts;
+  double tolerance;
+  bool stop_at_request;
+};
+// Duals with the reference's sign flip, status and message, dual bound, recompute_quality.
+// col_value must already hold the point; solve_seconds is left to the caller.
+void report(const Model& model, const double* res9, const std::vector<double>& y,
+            const std::vector<double>& reduced, const ReportState& rs, Solution* solution);
+
+// The device the gpu_devices option names (one GPU: the first listed, with a warning when
+// several are given).
+int engine_device(const Options& options, Logger& logger);
+
+// prepare() and power_start() for every model, on all host cores (one model per task).
+void prepare_all(const std::vector<const Model*>& models, std::int64_t random_seed,
+                 std::vector<Prepared>* prep, std::vector<std::vector<double>>* v0);
+
+// A batch's arrays, concatenated LP after LP in the layout include/pdhg_batch.h takes.
+struct BatchInput {
+  std::vector<int32_t> m, n;
+  std::vector<int64_t> nnz;
+  std::vector<int32_t> a_rp, a_ci, t_rp, t_ci;
+  std::vector<double> a_v, t_v, chat, lhat, uhat, rlhat, ruhat, dc, dr, cost, cl, cu, rl, ru;
+  std::vector<double> bound_norm, cost_norm, v0;
+};
+BatchInput make_batch_input(const std::vector<const Model*>& models,
+                            const std::vector<Prepared>& prep,
+                            const std::vector<std::vector<double>>& v0);
+
+}  // namespace sankhya::hip_engine

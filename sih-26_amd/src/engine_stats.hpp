@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <vector>
 
 #include "sankhya/logging.hpp"
 #include "sankhya/model.hpp"
@@ -19,7 +20,11 @@ struct EngineStats {
 };
 
 const EngineStats& last_engine_stats();
+// Per-LP counters of the last batched run on this thread, in model order.
+const std::vector<EngineStats>& last_batch_stats();
 
 Solution run(const Model& model, const Options& options, Logger& logger, SolveControl* control);
+std::vector<Solution> run_batch(const std::vector<const Model*>& models, const Options& options,
+                                Logger& logger, SolveControl* control);
 
 }  // namespace sankhya::hip_engine

@@ -63,6 +63,11 @@ std::vector<OptionSpec> build_registry() {
                "relative_kkt: stop when the relative KKT error, max(relative primal, dual, gap), "
                "is <= pdhg_tolerance; the report still only claims optimal on the standard.",
                0, 0, {"reference", "relative_kkt"}});
+  s.push_back({"pdhg_batch_tune", OptionType::Bool, true,
+               "Batched engine: time the three workgroup geometries on every LP and keep the "
+               "fastest per LP before solving.", 0, 0});
+  s.push_back({"pdhg_batch_tune_attempts", OptionType::Int, std::int64_t{96},
+               "Batched engine: attempts per geometry when tuning the schedule.", 1.0, 100000.0});
   s.push_back({"pdhg_chunk", OptionType::Int, std::int64_t{0},
                "Attempts per captured graph chunk; 0 picks the default.", 0.0, 4096.0});
   s.push_back({"log_level", OptionType::String, std::string("info"), "off, error, warning, info, verbose, debug.",

@@ -1,0 +1,221 @@
+// SPDX-License-Identifier: Apache-2.0
+// Host driver of the batched MI355X PDHG engine: sankhya::gpu::solve_pdhg_batch.
+//
+// Per LP it does what the single-LP driver does (src/pdhg/pdhg.cpp:76-192 setup, :486-567
+// report): validate, minimise-space cost and norms, preconditioning, the CSR of Ahat, the
+// power iteration's start vector. The preparation runs on all host cores, one LP per task. The
+// device part goes through include/pdhg_batch.h: upload once, per-LP spectral norms,
+// optionally the schedule tuner, then graph chunks until every LP has stopped or the time
+// limit / interrupt ends the batch.
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/pdhg_batch.h"
+#include "engine_common.hpp"
+#include "engine_stats.hpp"
+#include "sankhya/pdhg.hpp"
+#include "sankhya/timer.hpp"
+
+namespace sankhya {
+namespace hip_engine {
+
+thread_local std::vector<EngineStats> g_batch_stats;
+const std::vector<EngineStats>& last_batch_stats() { return g_batch_stats; }
+
+namespace {
+
+struct BatchHandle {
+  pdhg_batch* h = nullptr;
+  ~BatchHandle() { pdhg_batch_destroy(h); }
+};
+
+}  // namespace
+
+std::vector<Solution> run_batch(const std::vector<const Model*>& models, const Options& options,
+                                Logger& logger, SolveControl* control) {
+  Timer timer;
+  const std::size_t nlp = models.size();
+  std::vector<Solution> out(nlp);
+  g_batch_stats.assign(nlp, EngineStats{});
+  const bool mixed = options.get_string("pdhg_precision") == "mixed";
+  const std::string algorithm = mixed ? "pdhg-gpu-batch-mixed" : "pdhg-gpu-batch";
+  // Models that fail validation get kModelError and stay out of the device batch.
+  std::vector<std::size_t> live;
+  for (std::size_t k = 0; k < nlp; ++k) {
+    out[k].algorithm = algorithm;
+    if (!models[k]) {
+      out[k].status = SolveStatus::kModelError;
+      out[k].message = "null model";
+      continue;
+    }
+    out[k].allocate_for(*models[k]);
+    const std::string text = models[k]->validate();
+    if (!text.empty()) {
+      out[k].status = SolveStatus::kModelError;
+      out[k].message = text;
+      continue;
+    }
+    live.push_back(k);
+  }
+  if (live.empty()) return out;
+  auto fail_all = [&](const std::string& what) {
+    for (const std::size_t k : live) {
+      out[k].status = SolveStatus::kNumericalError;
+      out[k].message = "MI355X batched PDHG: " + what;
+      out[k].solve_seconds = timer.elapsed_seconds();
+    }
+    logger.error("MI355X batched PDHG: {}", what);
+    return out;
+  };
+  const int device = engine_device(options, logger);
+  char desc[256] = {0};
+  if (pdhg_dev_probe(device, desc, sizeof(desc)) != PDHG_OK)
+    return fail_all(std::string("no usable gfx950 device (") + desc + ")");
+
+  // ---- Host preparation, one LP per task ----
+  const std::size_t nb = live.size();
+  std::vector<const Model*> live_models;
+  for (const std::size_t k : live) live_models.push_back(models[k]);
+  std::vector<Prepared> prep;
+  std::vector<std::vector<double>> v0;
+  prepare_all(live_models, options.get_int("random_seed"), &prep, &v0);
+  const BatchInput in = make_batch_input(live_models, prep, v0);
+  const std::vector<int32_t>& m = in.m;
+  const std::vector<int32_t>& n = in.n;
+  const double setup_seconds = timer.elapsed_seconds();
+
+  BatchHandle dev;
+  int rc = pdhg_batch_create(device, mixed ? PDHG_PRECISION_MIXED : PDHG_PRECISION_FP64, &dev.h);
+  if (rc != PDHG_OK) return fail_all("batch handle creation failed (code " + std::to_string(rc) + ")");
+  auto dev_error = [&](int code, const char* step) {
+    return fail_all(std::string(step) + " failed (code " + std::to_string(code) + "): " +
+                    pdhg_batch_last_error(dev.h));
+  };
+  rc = pdhg_batch_upload(dev.h, static_cast<int32_t>(nb), in.m.data(), in.n.data(), in.nnz.data(),
+                         in.a_rp.data(), in.a_ci.data(), in.a_v.data(), in.t_rp.data(),
+                         in.t_ci.data(), in.t_v.data(), in.chat.data(), in.lhat.data(),
+                         in.uhat.data(), in.rlhat.data(), in.ruhat.data(), in.dc.data(),
+                         in.dr.data(), in.cost.data(), in.cl.data(), in.cu.data(), in.rl.data(),
+                         in.ru.data(), in.bound_norm.data(), in.cost_norm.data());
+  if (rc != PDHG_OK) return dev_error(rc, "upload");
+  std::vector<double> norms(nb, 1.0);
+  rc = pdhg_batch_spectral_norms(dev.h, in.v0.data(), kPowerIterations, norms.data());
+  if (rc != PDHG_OK) return dev_error(rc, "spectral norms");
+
+  const double tolerance = options.get_double("pdhg_tolerance");
+  const std::int64_t limit_option = options.get_int("iteration_limit");
+  const Count iteration_limit = limit_option < 0 ? Count{1000000} : static_cast<Count>(limit_option);
+  const bool stop_at_request = options.get_bool("pdhg_stop_at_request");
+  const double time_limit = options.get_double("time_limit");
+  pdhg_params params{};
+  params.tolerance = tolerance;
+  params.iteration_limit = iteration_limit;
+  params.restart = options.get_bool("pdhg_restart") ? 1 : 0;
+  params.stop_at_request = stop_at_request ? 1 : 0;
+  params.spectral_norm = 1.0;
+  params.chunk_attempts = static_cast<int32_t>(options.get_int("pdhg_chunk"));
+  params.stop_rule = options.get_string("pdhg_stop_rule") == "relative_kkt" ? 1 : 0;
+  logger.info("Solving {} LPs with MI355X batched PDHG on {}, precision {}", nb, desc,
+              mixed ? "mixed" : "fp64");
+  if (options.get_bool("pdhg_batch_tune")) {
+    std::vector<int32_t> geo(nb, 0);
+    rc = pdhg_batch_tune(dev.h, &params, norms.data(),
+                         static_cast<int>(options.get_int("pdhg_batch_tune_attempts")), geo.data());
+    if (rc != PDHG_OK) return dev_error(rc, "schedule tuning");
+    int count[3] = {0, 0, 0}, coop = 0;
+    for (const int32_t g : geo) {
+      if (g < 0) ++coop;
+      else ++count[std::min(g, 2)];
+    }
+    logger.info("Schedule: {} cooperative, {} persistent at 256 threads, {} at 512, {} at 1024", coop, count[0], count[1], count[2]);
+  }
+  rc = pdhg_batch_start(dev.h, &params, norms.data());
+  if (rc != PDHG_OK) return dev_error(rc, "start");
+  const double loop_start = timer.elapsed_seconds();
+  SolveStatus stop_status = SolveStatus::kIterationLimit;
+  int32_t running = static_cast<int32_t>(nb);
+  while (running > 0) {
+    if (std::isfinite(time_limit) && timer.elapsed_seconds() >= time_limit) {
+      stop_status = SolveStatus::kTimeLimit;
+      break;
+    }
+    if (control && control->interruption_requested()) {
+      stop_status = SolveStatus::kInterrupted;
+      break;
+    }
+    rc = pdhg_batch_run(dev.h, 1, &running);
+    if (rc != PDHG_OK) return dev_error(rc, "iteration");
+  }
+  const double loop_seconds = timer.elapsed_seconds() - loop_start;
+
+  // ---- Report, per LP ----
+  std::int64_t sum_n = 0, sum_m = 0;
+  for (std::size_t q = 0; q < nb; ++q) {
+    sum_n += n[q];
+    sum_m += m[q];
+  }
+  std::vector<double> x(static_cast<std::size_t>(sum_n)), y(static_cast<std::size_t>(sum_m)),
+      red(static_cast<std::size_t>(sum_n)), res(9 * nb);
+  std::vector<pdhg_state> states(nb);
+  rc = pdhg_batch_finish(dev.h, x.data(), y.data(), red.data(), res.data());
+  if (rc != PDHG_OK) return dev_error(rc, "report");
+  rc = pdhg_batch_states(dev.h, states.data());
+  if (rc != PDHG_OK) return dev_error(rc, "state read");
+  const double total = timer.elapsed_seconds();
+  std::int64_t off_n = 0, off_m = 0;
+  for (std::size_t q = 0; q < nb; ++q) {
+    const std::size_t k = live[q];
+    Solution& s = out[k];
+    std::copy(x.begin() + off_n, x.begin() + off_n + n[q], s.col_value.begin());
+    std::vector<double> yl(y.begin() + off_m, y.begin() + off_m + m[q]);
+    std::vector<double> rl2(red.begin() + off_n, red.begin() + off_n + n[q]);
+    const pdhg_state& st = states[q];
+    s.solve_seconds = total;
+    EngineStats& es = g_batch_stats[k];
+    es.iterations = st.iteration;
+    es.attempts = st.attempts;
+    es.restarts = st.restarts;
+    es.ticks = st.ticks;
+    es.device_bytes = pdhg_batch_bytes(dev.h);
+    es.spectral_norm = norms[q];
+    es.setup_seconds = setup_seconds;
+    es.loop_seconds = loop_seconds;
+    es.total_seconds = total;
+    es.final_primal = res[9 * q + 0];
+    es.final_dual = res[9 * q + 1];
+    es.final_gap = res[9 * q + 2];
+    report(*models[k], res.data() + 9 * q, yl, rl2,
+           ReportState{st.done != 0, st.done ? SolveStatus::kOptimal : stop_status, st.iteration,
+                       st.restarts, tolerance, stop_at_request},
+           &s);
+    off_n += n[q];
+    off_m += m[q];
+  }
+  logger.info("Batch of {} LPs: setup {:.3f}s, iterations {:.3f}s, total {:.3f}s", nb, setup_seconds,
+              loop_seconds, total);
+  return out;
+}
+
+}  // namespace hip_engine
+
+namespace gpu {
+std::vector<Solution> solve_pdhg_batch(const std::vector<const Model*>& models,
+                                       const Options& options, Logger& logger,
+                                       SolveControl* control) {
+  try {
+    return hip_engine::run_batch(models, options, logger, control);
+  } catch (const std::exception& e) {
+    std::vector<Solution> out(models.size());
+    for (Solution& s : out) {
+      s.status = SolveStatus::kNumericalError;
+      s.algorithm = "pdhg-gpu-batch";
+      s.message = std::string("batched PDHG failed: ") + e.what();
+    }
+    return out;
+  }
+}
+}  // namespace gpu
+
+}  // namespace sankhya

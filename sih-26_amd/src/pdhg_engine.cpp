@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/pdhg_hip.h"
+#include "engine_common.hpp"
 #include "engine_stats.hpp"
 #include "sankhya/pdhg.hpp"
 #include "sankhya/timer.hpp"
@@ -32,68 +33,13 @@ const EngineStats& last_engine_stats() { return g_last_stats; }
 
 namespace {
 
-constexpr int kRuizIterations = 10;   // src/pdhg/pdhg.cpp:62
-constexpr int kPowerIterations = 30;  // src/pdhg/pdhg.cpp:63
-
-struct Residuals {
-  double primal = 0, dual = 0, gap = 0, primal_objective = 0, dual_objective = 0;
-  double absolute_primal = 0, absolute_dual = 0, gap_as_verified = 0, complementarity = 0;
-  [[nodiscard]] bool meets_standard() const {
-    return absolute_primal <= tol::kPrimalFeasibility && absolute_dual <= tol::kDualFeasibility &&
-           gap_as_verified <= tol::kDualityGap && complementarity <= 1e-6;
-  }
-};
-
-Residuals from_array(const double* r) {
-  Residuals x;
-  x.primal = r[0];
-  x.dual = r[1];
-  x.gap = r[2];
-  x.primal_objective = r[3];
-  x.dual_objective = r[4];
-  x.absolute_primal = r[5];
-  x.absolute_dual = r[6];
-  x.gap_as_verified = r[7];
-  x.complementarity = r[8];
-  return x;
-}
-
-// CSR of a CSC pattern by a counting-sort transpose: column indices ascend within each row
-// (reference CsrView::build, src/la/sparse.cpp:271-322).
-void csc_to_csr(Index m, Index n, const std::vector<Index>& cp, const std::vector<Index>& ri,
-                const std::vector<double>& v, std::vector<Index>* rp, std::vector<Index>* ci,
-                std::vector<double>* cv) {
-  const std::size_t nnz = v.size();
-  rp->assign(static_cast<std::size_t>(m) + 1, 0);
-  for (std::size_t k = 0; k < nnz; ++k) ++(*rp)[static_cast<std::size_t>(ri[k]) + 1];
-  for (Index i = 0; i < m; ++i) (*rp)[i + 1] += (*rp)[i];
-  ci->resize(nnz);
-  cv->resize(nnz);
-  std::vector<Index> next(rp->begin(), rp->end() - 1);
-  for (Index j = 0; j < n; ++j) {
-    for (Index k = cp[j]; k < cp[j + 1]; ++k) {
-      const auto slot = static_cast<std::size_t>(next[ri[k]]++);
-      (*ci)[slot] = j;
-      (*cv)[slot] = v[k];
-    }
-  }
-}
-
-int device_from_option(const std::string& text, Logger& logger) {
-  if (text.empty() || text == "auto") return 0;
-  const std::size_t comma = text.find(',');
-  if (comma != std::string::npos)
-    logger.warning("gpu_devices lists several devices; this engine runs on one GPU and uses {}",
-                   text.substr(0, comma));
-  return std::atoi(text.substr(0, comma).c_str());
-}
-
 struct DevHandle {
   pdhg_dev* h = nullptr;
   ~DevHandle() { pdhg_dev_destroy(h); }
 };
 
 }  // namespace
+
 
 Solution run(const Model& model, const Options& options, Logger& logger, SolveControl* control) {
   Timer timer;
@@ -117,7 +63,7 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
     return solution;
   };
 
-  const int device = device_from_option(options.get_string("gpu_devices"), logger);
+  const int device = engine_device(options, logger);
   char desc[256] = {0};
   if (pdhg_dev_probe(device, desc, sizeof(desc)) != PDHG_OK)
     return device_failure(std::string("no usable gfx950 device (") + desc + ")");
@@ -126,27 +72,9 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   const double sense = model.sense_multiplier();
   const auto n = static_cast<std::size_t>(cols), m = static_cast<std::size_t>(rows);
 
-  // ---- The unscaled problem, in minimise space (pdhg.cpp:94-111) ----
-  std::vector<double> cost(n);
-  for (std::size_t j = 0; j < n; ++j) cost[j] = sense * model.col_cost[j];
-  double cost_square = 0.0;
-  for (const double c : cost) cost_square += c * c;
-  const double cost_norm = std::sqrt(cost_square);
-  double bound_square = 0.0;
-  for (std::size_t i = 0; i < m; ++i) {
-    const double b = is_finite_bound(model.row_lower[i])
-                         ? model.row_lower[i]
-                         : (is_finite_bound(model.row_upper[i]) ? model.row_upper[i] : 0.0);
-    bound_square += b * b;
-  }
-  const double bound_norm = std::sqrt(bound_square);
-
-  // ---- Preconditioning (pdhg.cpp:113-117) and the two CSR layouts the device reads ----
-  const Scaling scaling = build_scaling(model, cost, kRuizIterations);
-  std::vector<Index> a_rp, a_ci;
-  std::vector<double> a_v;
-  csc_to_csr(rows, cols, model.matrix.column_starts(), model.matrix.row_indices(), scaling.values,
-             &a_rp, &a_ci, &a_v);
+  const Prepared prep = prepare(model);
+  const Scaling& scaling = prep.scaling;
+  const std::vector<double>& cost = prep.cost;
   g_last_stats.setup_seconds = timer.elapsed_seconds();
 
   DevHandle dev;
@@ -158,22 +86,17 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
     return device_failure(std::string(step) + " failed (code " + std::to_string(code) + "): " +
                           pdhg_dev_last_error(dev.h));
   };
-  rc = pdhg_dev_upload(dev.h, rows, cols, model.num_nonzeros(), a_rp.data(), a_ci.data(), a_v.data(),
+  rc = pdhg_dev_upload(dev.h, rows, cols, model.num_nonzeros(), prep.a_rp.data(), prep.a_ci.data(),
+                       prep.a_v.data(),
                        model.matrix.column_starts().data(), model.matrix.row_indices().data(),
                        scaling.values.data(), scaling.cost.data(), scaling.col_lower.data(),
                        scaling.col_upper.data(), scaling.row_lower.data(), scaling.row_upper.data(),
                        scaling.column.data(), scaling.row.data(), cost.data(), model.col_lower.data(),
                        model.col_upper.data(), model.row_lower.data(), model.row_upper.data(),
-                       bound_norm, cost_norm);
+                       prep.bound_norm, prep.cost_norm);
   if (rc != PDHG_OK) return dev_error(rc, "upload");
 
-  // Power iteration start vector: std::mt19937(random_seed + 1), U(-1, 1) (scaling.cpp:168-171).
-  std::vector<double> v0(n);
-  {
-    std::mt19937 rng(static_cast<unsigned>(options.get_int("random_seed")) + 1u);
-    std::uniform_real_distribution<double> spread(-1.0, 1.0);
-    for (double& v : v0) v = spread(rng);
-  }
+  const std::vector<double> v0 = power_start(model, options.get_int("random_seed"));
   double spectral_norm = 1.0;
   rc = pdhg_dev_spectral_norm(dev.h, v0.data(), kPowerIterations, &spectral_norm);
   if (rc != PDHG_OK) return dev_error(rc, "spectral norm");
@@ -318,9 +241,6 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   rc = pdhg_dev_finish(dev.h, solution.col_value.data(), y.data(), reduced.data(), res);
   if (rc != PDHG_OK) return dev_error(rc, "report");
   const Residuals final_r = from_array(res);
-  for (std::size_t j = 0; j < n; ++j) solution.col_dual[j] = sense * reduced[j];
-  for (std::size_t i = 0; i < m; ++i) solution.row_dual[i] = sense * (-y[i]);
-  solution.iterations = st.iteration;
   g_last_stats.iterations = st.iteration;
   g_last_stats.attempts = st.attempts;
   g_last_stats.restarts = st.restarts;
@@ -329,43 +249,10 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   g_last_stats.final_primal = final_r.primal;
   g_last_stats.final_dual = final_r.dual;
   g_last_stats.final_gap = final_r.gap;
-
-  const bool verifiable = converged && final_r.meets_standard();
-  if (verifiable) {
-    solution.status = SolveStatus::kOptimal;
-    solution.message = detail::format(
-        "MI355X PDHG converged after {} iterations and {} restarts; absolute primal {:.3e}, dual "
-        "{:.3e}, relative gap {:.3e}",
-        st.iteration, st.restarts, final_r.absolute_primal, final_r.absolute_dual,
-        final_r.gap_as_verified);
-  } else if (converged) {
-    solution.status = SolveStatus::kFeasible;
-    solution.message = detail::format(
-        "met the requested relative tolerance {:.1e} after {} iterations, but NOT the absolute "
-        "standard this project verifies against (primal {:.3e} vs {:.1e}, dual {:.3e} vs {:.1e}, "
-        "relative gap {:.3e} vs {:.1e}). Reported as feasible, not optimal. {}",
-        tolerance, st.iteration, final_r.absolute_primal, tol::kPrimalFeasibility,
-        final_r.absolute_dual, tol::kDualFeasibility, final_r.gap_as_verified, tol::kDualityGap,
-        stop_at_request ? "pdhg_stop_at_request is on, so this is the cheap answer that was asked "
-                          "for; turn it off to run on to the standard"
-                        : "Tighten --option pdhg_tolerance to close it");
-  } else {
-    solution.status = (stop_status == SolveStatus::kTimeLimit || stop_status == SolveStatus::kInterrupted)
-                          ? stop_status
-                          : SolveStatus::kIterationLimit;
-    solution.message = detail::format(
-        "stopped at relative primal {:.3e}, dual {:.3e}, gap {:.3e} after {} iterations and {} "
-        "restarts (target {:.1e})",
-        final_r.primal, final_r.dual, final_r.gap, st.iteration, st.restarts, tolerance);
-  }
-  solution.stopped_by = solution.status == SolveStatus::kTimeLimit     ? LimitReason::kTime
-                        : solution.status == SolveStatus::kInterrupted ? LimitReason::kInterrupt
-                        : solution.status == SolveStatus::kIterationLimit ? LimitReason::kIterations
-                                                                           : LimitReason::kNone;
-  solution.dual_bound = verifiable ? sense * final_r.dual_objective + model.objective_offset
-                                   : (model.sense == ObjSense::kMaximize ? kInfinity : -kInfinity);
   solution.solve_seconds = timer.elapsed_seconds();
-  solution.recompute_quality(model);
+  report(model, res, y, reduced,
+         ReportState{converged, stop_status, st.iteration, st.restarts, tolerance, stop_at_request},
+         &solution);
   g_last_stats.total_seconds = timer.elapsed_seconds();
 
   logger.info("");

@@ -86,3 +86,16 @@ KNOWN_ANSWERS = [
      lambda: dense_lp([[1, 0], [0, 2], [3, 2]], [-INF] * 3, [4, 12, 18], [3, 5], maximize=True), 36.0, 1e-5),
     ("CarriesTheObjectiveOffset", lambda: dense_lp([[1]], [3], [INF], [1], offset=10.0), 13.0, 1e-6),
 ]
+
+
+def c3_sizes(count: int, seed: int = 1000):
+    """Row counts of the batched workload: m log-uniform in [200, 20000] (rounded), n = 2.5 m."""
+    rng = np.random.default_rng(seed)
+    m = np.rint(np.exp(rng.uniform(np.log(200.0), np.log(20000.0), count))).astype(int)
+    return [(int(mi), int(round(2.5 * mi))) for mi in m]
+
+
+def c3_batch(count: int, seed: int = 1000):
+    """`count` LPs of the batched workload: the 1,000 x 3,000 instance's generator (uniform rows,
+    equality constraints, 6 nonzeros per column) at c3_sizes, LP i seeded seed + i."""
+    return [sh.Model.generate("uniform", m, n, 6, seed + i) for i, (m, n) in enumerate(c3_sizes(count, seed))]

@@ -1,0 +1,105 @@
+"""The batched engine (include/pdhg_batch.h) on a gfx950 device, through the C ABI.
+
+Each LP in a batch runs the single-LP algorithm, so parity is checked the same way as for the
+single-LP engine, per LP:
+  - known answers of the reference's tests/unit/test_pdhg.cpp, all in one batch;
+  - Netlib instances in one batch at tolerance 1e-8, with and without the schedule tuner,
+    against the oracle (same status, objective within 1e-6 relative), the published optima and
+    tests/verify.py; three 1,000 x 3,000 instances in one batch at 1e-6 (stop at request);
+  - mixed precision on a sample of the batched workload (m log-uniform in [200, 20000],
+    n = 2.5 m) against FP64 single-LP solves of the same LPs at 1e-6 relative KKT (objective
+    within 1e-5 relative: both are 1e-6-accurate points);
+  - the schedule tuner returns a geometry for every LP and does not change answers.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import common as cm
+import oracle as orc
+import sankhya_hip as sh
+from verify import accepted
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    desc = cm.gpu_or_skip()
+    print(f"[gpu] {desc}", flush=True)
+
+
+def _sol(s):
+    return dict(status=s.status, objective=s.objective, col_value=s.col_value, row_dual=s.row_dual,
+                col_dual=s.col_dual, row_activity=s.row_activity)
+
+
+def test_batch_known_answers():
+    models = [cm.model_of(make()) for _, make, _, _ in cm.KNOWN_ANSWERS]
+    sols = sh.solve_batch(models, pdhg_tolerance=1e-8, iteration_limit=200000, pdhg_polish=False,
+                          pdhg_batch_tune=False)
+    for (name, _, expected, tol), s in zip(cm.KNOWN_ANSWERS, sols):
+        assert s.status == "optimal", (name, s.message)
+        assert abs(s.objective - expected) <= tol, (name, s.objective)
+        assert s.algorithm == "pdhg-gpu-batch"
+
+
+def _check_against_oracle(data, sols, names, tol, stop_at_request):
+    for d, s, nm in zip(data, sols, names):
+        r = orc.solve(orc.Problem(d), tolerance=tol, iteration_limit=1000000,
+                      stop_at_request=stop_at_request)
+        assert s.status == r["status"], (nm, s.status, r["status"], s.message)
+        assert s.status in ("optimal", "feasible"), (nm, s.message)
+        rel = abs(s.objective - r["objective"]) / max(1.0, abs(r["objective"]))
+        print(f"  {nm}: it={s.iterations} oracle it={r['iterations']} rel={rel:.1e}", flush=True)
+        assert rel <= 1e-6, (nm, s.objective, r["objective"])
+        ok, checks = accepted(d, _sol(s))
+        assert ok, (nm, checks)
+
+
+@pytest.mark.parametrize("tune", [False, True])
+def test_batch_of_netlib_agrees_with_oracle_at_1e8(tune):
+    names = ["afiro", "sc50a", "sc50b", "blend", "sc105", "adlittle", "stocfor1"]
+    data = [cm.netlib(nm) for nm in names]
+    t = time.time()
+    sols = sh.solve_batch([cm.model_of(d) for d in data], pdhg_tolerance=1e-8,
+                          iteration_limit=1000000, pdhg_polish=False, pdhg_batch_tune=tune)
+    print(f"batch of {len(data)} Netlib LPs (tune={tune}) in {time.time()-t:.2f}s", flush=True)
+    _check_against_oracle(data, sols, names, 1e-8, False)
+    for nm, d, s in zip(names, data, sols):
+        single = sh.solve(cm.model_of(d), iteration_limit=1).stats["spectral_norm"]
+        assert abs(s.stats["spectral_norm"] - single) <= 1e-12 * single, (nm, s.stats["spectral_norm"], single)
+        pub = cm.netlib_optima()[nm]["published_optimal"]
+        assert abs(s.objective - pub) / max(1.0, abs(pub)) < 1e-6, (nm, s.objective, pub)
+
+
+def test_batch_of_c0_seeds_agrees_with_oracle_at_1e6():
+    # Seed 3 is left out: under the absolute-primal clause of the reference's test it takes the
+    # oracle 987k iterations, so trajectories that differ by rounding end on either side of 1e6.
+    pairs = [cm.c0(seed) for seed in (1, 2, 4)]
+    t = time.time()
+    sols = sh.solve_batch([p[0] for p in pairs], pdhg_tolerance=1e-6, pdhg_stop_at_request=True,
+                          pdhg_polish=False)
+    print(f"batch of 3 C0 LPs in {time.time()-t:.2f}s", flush=True)
+    _check_against_oracle([p[1] for p in pairs], sols, ["c0-1", "c0-2", "c0-4"], 1e-6, True)
+
+
+def test_mixed_batch_sample_of_the_batched_workload_against_fp64_single_solves():
+    models = cm.c3_batch(24, seed=1000)
+    opts = dict(pdhg_tolerance=1e-6, pdhg_stop_rule="relative_kkt", iteration_limit=400000)
+    t = time.time()
+    sols = sh.solve_batch(models, pdhg_precision="mixed", **opts)
+    tb = time.time() - t
+    t = time.time()
+    ref = [sh.solve(md, **opts) for md in models]
+    ts = time.time() - t
+    worst = 0.0
+    for md, s, r in zip(models, sols, ref):
+        assert s.status == r.status == "feasible", (md.dims, s.status, r.status, s.message)
+        assert max(s.stats["final_primal"], s.stats["final_dual"], s.stats["final_gap"]) <= 1e-6
+        rel = abs(s.objective - r.objective) / max(1.0, abs(r.objective))
+        worst = max(worst, rel)
+        assert rel <= 1e-5, (md.dims, s.objective, r.objective)
+    print(f"24 LPs: mixed batch {tb:.2f}s, fp64 one by one {ts:.2f}s ({ts/tb:.1f}x); worst "
+          f"relative objective difference {worst:.1e}", flush=True)

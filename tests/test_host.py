@@ -33,7 +33,7 @@ def declared_functions(header):
     return names
 
 
-@pytest.mark.parametrize("header", ["pdhg_hip.h", "sankhya_pdhg.h"])
+@pytest.mark.parametrize("header", ["pdhg_hip.h", "pdhg_batch.h", "sankhya_pdhg.h"])
 def test_library_exports_every_declared_function(header):
     lib = sh.load_library()
     names = declared_functions(os.path.join(ROOT, "include", header))
