@@ -84,6 +84,22 @@ int pdhg_batch_states(pdhg_batch* h, pdhg_state* out);
 int pdhg_batch_finish(pdhg_batch* h, double* col_value, double* y, double* reduced,
                       double* residuals);
 
+/* The adjoint (no reference counterpart): vector-Jacobian products of each LP's solution map
+ * at its reported best point, by implicit differentiation of the PDHG fixed point on its
+ * active set with CGLS on the masked matrix, run one workgroup per LP in FP64. The active set
+ * is read by complementarity: a column is free when its distance to its nearest bound exceeds
+ * |c + A^T y|; a row is active when it is an equality or |y| exceeds its distance to its nearest
+ * bound. Inputs, concatenated like the solutions: grad_x = dL/dx (sum n) and grad_y =
+ * dL/dy (sum m) for the PDHG multiplier y (the reported row dual is sense * (-y)). Outputs:
+ * grad_b (sum m) = dL/d(the active bound of each active row; both bounds of an equality row
+ * move together), zero on inactive rows; grad_c (sum n) = dL/d(minimise-space cost), zero on
+ * columns at a bound. tol: relative residual of the normal equations at which CGLS stops;
+ * iterations and residuals (2 per LP: the x and the y solve; may be NULL). Needs a started
+ * batch whose run has ended (pdhg_batch_finish not required). */
+int pdhg_batch_adjoint(pdhg_batch* h, const double* grad_x, const double* grad_y, double tol,
+                       int max_iterations, double* grad_b, double* grad_c, int32_t* iterations,
+                       double* residuals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,28 @@ int sankhya_pdhg_solve_batch(const sankhya_model* const* models, int32_t nlp, co
                              double* col_value, double* row_dual, double* col_dual,
                              double* row_activity, sankhya_result* results);
 
+/* A solved batch kept on the device for vector-Jacobian products (the backward pass of a
+ * differentiable LP layer). sankhya_batch_solve solves like sankhya_pdhg_solve_batch and returns
+ * the session (NULL when an option is rejected; results[0].message says which). */
+typedef struct sankhya_batch sankhya_batch;
+sankhya_batch* sankhya_batch_solve(const sankhya_model* const* models, int32_t nlp,
+                                   const char* options, double* col_value, double* row_dual,
+                                   double* col_dual, double* row_activity,
+                                   sankhya_result* results);
+
+/* dL/dc (sum n) and dL/d(active row bound) (sum m) of every LP for upstream dL/d(col_value)
+ * (sum n) and dL/d(row_dual) (sum m), in the models' own sense and sign conventions; inactive
+ * rows and columns at a bound get 0. Implicit differentiation of the PDHG fixed point on the
+ * active set, FP64 CGLS on the device (include/pdhg_batch.h, pdhg_batch_adjoint): tol is the
+ * relative normal-equation residual at which it stops. iterations and residuals: 2 per LP (may
+ * be NULL). Returns 0 or a PDHG_E_* code. */
+int sankhya_batch_adjoint(const sankhya_batch* session, const double* grad_col_value,
+                          const double* grad_row_dual, double tol, int max_iterations,
+                          double* grad_cost, double* grad_row_bound, int32_t* iterations,
+                          double* residuals);
+
+void sankhya_batch_free(sankhya_batch* session);
+
 /* The host preconditioner on its own (src/la/scaling.cpp:111-159): Dr (m), Dc (n), the scaled
  * CSC values (nnz), scaled cost/bounds. min_cost is sense * c. */
 int sankhya_pdhg_scaling(const sankhya_model* model, int ruiz_iterations, double* row,

@@ -613,6 +613,198 @@ __global__ __launch_bounds__(GEO::kThreads) void k_batch_power(Args<double, doub
   }
 }
 
+// ---- the adjoint ------------------------------------------------------------------------
+// Implicit differentiation of the PDHG fixed point. At the solution the projections'
+// Jacobians are 0/1 masks: free columns F (strictly inside their bounds) and active rows R
+// (equalities, and rows whose multiplier is nonzero, i.e. at a bound). Linearised on that active
+// set the fixed point is Ahat_RF xhat_F = bhat_R - Ahat_RN xhat_N and Ahat_RF^T yhat_R = -chat_F,
+// so the vector-Jacobian products of the solution map are two least-squares solves with the
+// masked matrix M = Ahat_RF, done by CGLS on the same two products PDHG uses, swapped:
+//   x side: M^T lambda = Dc gx_F            ->  dL/db_R = Dr lambda
+//   y side: M mu = h_R, h = Dr gy          ->  dL/dc_F (minimise space) = -Dc mu
+// (gx = dL/dx, gy = dL/dy for the PDHG multiplier y; least squares gives the minimum-norm
+// answer where the active set is degenerate). Everything is FP64 and in scaled space.
+
+// Masked product epilogue: out[i] = mask[i] * sum, plus the sum of squares written.
+struct MaskEpi {
+  double* out;
+  const double* mask;
+  struct In {
+    double k;
+  };
+  __device__ In load(int i) const { return In{mask[i]}; }
+  __device__ void operator()(int i, const RowSum<1>& s, const In& in, Acc<1, 0>& a) const {
+    const double v = in.k * s.v[0];
+    out[i] = v;
+    a.s[0] += v * v;
+  }
+};
+
+// Reduced cost of the reported point: d_j = cost_j + (A^T y)_j = cost_j + (Ahat^T yhat)_j / Dc_j.
+struct RedCostEpi {
+  double* out;
+  const double *cost, *dc;
+  struct In {
+    double c, dc;
+  };
+  __device__ In load(int j) const { return In{cost[j], dc[j]}; }
+  __device__ void operator()(int j, const RowSum<1>& s, const In& in, Acc<1, 0>&) const {
+    out[j] = in.c + s.v[0] / in.dc;
+  }
+};
+
+struct AdjWork {
+  double *mask_c, *mask_r;      // active set (sum n, sum m)
+  const double *gx, *gy;        // upstream gradients (sum n, sum m), unscaled
+  double *gb, *gc;              // results (sum m, sum n), unscaled; gc in minimise space
+  double *vn0, *vn1, *vn2;      // n-sized CGLS vectors
+  double *vm0, *vm1, *vm2;      // m-sized CGLS vectors
+  int32_t* iters;               // 2 per LP: CGLS iterations of the x and y solves
+  double* resid;                // 2 per LP: final relative normal-equation residuals
+};
+
+// One product of the masked matrix: out = mask_out * (P in), returns ||out||^2 (all lanes).
+template <class GEO>
+__device__ double masked_product(const PlanView<double>& P, const double* in, double* out,
+                                 const double* mask_out, double* lds, double* red,
+                                 double* bcast) {
+  PlainGather g{in, 0.0, 0.0};
+  MaskEpi e{out, mask_out};
+  Acc<1, 0> a;
+  a.zero();
+  walk<GEO, double, 1>(P, g, e, a, lds, red);
+  block_reduce(a, red);
+  if (threadIdx.x == 0) *bcast = a.s[0];
+  __syncthreads();
+  return *bcast;
+}
+
+// CGLS for min || B z - g ||: B = Ahat_RF^T (P1 = Ahat^T plan, P2 = Ahat plan) or B = Ahat_RF
+// (the two plans swapped). z and p live on B's columns, r and q on its rows. Returns the
+// iterations; *rel receives ||B^T r|| / ||B^T g||.
+template <class GEO>
+__device__ int cgls(const PlanView<double>& PB, const PlanView<double>& PBt,
+                    const double* mask_rows_B, const double* mask_cols_B, int rows_B, int cols_B,
+                    double* z, double* r, double* q, double* p, double* sv, double tol,
+                    int maxit, double* lds, double* red, double* bcast, double* rel) {
+  constexpr int NT = GEO::kThreads;
+  // r = g (given), z = 0, s = B^T r, p = s.
+  for (int k = threadIdx.x; k < cols_B; k += NT) z[k] = 0.0;
+  __syncthreads();
+  double gamma = masked_product<GEO>(PBt, r, sv, mask_cols_B, lds, red, bcast);
+  const double gamma0 = gamma;
+  for (int k = threadIdx.x; k < cols_B; k += NT) p[k] = sv[k];
+  __syncthreads();
+  int it = 0;
+  double g_now = gamma;
+  while (it < maxit && gamma0 > 0.0 && g_now > tol * tol * gamma0) {
+    const double delta = masked_product<GEO>(PB, p, q, mask_rows_B, lds, red, bcast);
+    if (!(delta > 0.0)) break;
+    const double alpha = gamma / delta;
+    for (int k = threadIdx.x; k < cols_B; k += NT) z[k] += alpha * p[k];
+    for (int k = threadIdx.x; k < rows_B; k += NT) r[k] -= alpha * q[k];
+    __syncthreads();
+    g_now = masked_product<GEO>(PBt, r, sv, mask_cols_B, lds, red, bcast);
+    const double beta = g_now / gamma;
+    gamma = g_now;
+    for (int k = threadIdx.x; k < cols_B; k += NT) p[k] = sv[k] + beta * p[k];
+    __syncthreads();
+    ++it;
+  }
+  *rel = gamma0 > 0.0 ? sqrt(g_now / gamma0) : 0.0;
+  return it;
+}
+
+template <class GEO, int G>
+__global__ __launch_bounds__(GEO::kThreads) void k_batch_adjoint(Args<double, double> B,
+                                                                AdjWork W, double tol,
+                                                                int maxit) {
+  constexpr int NT = GEO::kThreads;
+  __shared__ double lds[lds_doubles<GEO>()];
+  __shared__ double red[(NT / 64) * 9];
+  __shared__ double bcast;
+  const int lp = B.order[blockIdx.x];
+  const LpDesc& d = B.desc[lp];
+  const PlanView<double> A64 = lp_view(B.A64, B.pa, d.a[G], d.nnz0, d.arp0);
+  const PlanView<double> T64 = lp_view(B.T64, B.pt, d.t[G], d.nnz0, d.trp0);
+  const Iter<double> it = lp_iter(B.it, d);
+  const int n = d.n, m = d.m;
+  double* mc = W.mask_c + d.col0;
+  double* mr = W.mask_r + d.row0;
+  // Active set from the reported (unscaled) best point, by complementarity: a column is free
+  // when its distance to its nearest bound exceeds its reduced cost |c + A^T y|, and a row is
+  // active when it is an equality or its multiplier exceeds its distance to its nearest bound.
+  double* m0 = W.vm0 + d.row0;
+  double* n0 = W.vn0 + d.col0;
+  for (int i = threadIdx.x; i < m; i += NT) m0[i] = it.besty[i] / it.dr[i];
+  __syncthreads();
+  {
+    PlainGather g{m0, 0.0, 0.0};
+    RedCostEpi e{n0, it.cost, it.dc};
+    Acc<1, 0> a;
+    a.zero();
+    walk<GEO, double, 1>(T64, g, e, a, lds, red);
+  }
+  for (int j = threadIdx.x; j < n; j += NT) {
+    const double x = it.bestx[j], lo = it.cl[j], hi = it.cu[j];
+    double slack = __builtin_huge_val();
+    if (finite_bound(lo)) slack = smin(slack, x - lo);
+    if (finite_bound(hi)) slack = smin(slack, hi - x);
+    mc[j] = slack > fabs(n0[j]) ? 1.0 : 0.0;
+  }
+  {
+    // Row activities for the row slacks: act = A x = Dr^-1 Ahat (x / Dc).
+    double* n1 = W.vn1 + d.col0;
+    for (int j = threadIdx.x; j < n; j += NT) n1[j] = it.bestx[j] / it.dc[j];
+    __syncthreads();
+    PlainGather g{n1, 0.0, 0.0};
+    PlainEpi e{m0};
+    Acc<1, 0> a;
+    a.zero();
+    walk<GEO, double, 1>(A64, g, e, a, lds, red);
+  }
+  for (int i = threadIdx.x; i < m; i += NT) {
+    const double act = m0[i] / it.dr[i], lo = it.rl[i], hi = it.ru[i];
+    double slack = __builtin_huge_val();
+    if (finite_bound(lo)) slack = smin(slack, act - lo);
+    if (finite_bound(hi)) slack = smin(slack, hi - act);
+    mr[i] = (lo == hi || fabs(it.besty[i]) > slack) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  double* gb = W.gb + d.row0;
+  double* gc = W.gc + d.col0;
+  for (int i = threadIdx.x; i < m; i += NT) gb[i] = 0.0;
+  for (int j = threadIdx.x; j < n; j += NT) gc[j] = 0.0;
+  double* n1 = W.vn1 + d.col0;
+  double* n2 = W.vn2 + d.col0;
+  double* m1 = W.vm1 + d.row0;
+  double* m2 = W.vm2 + d.row0;
+  // x side: B = M^T (rows: F columns of Ahat, cols: R rows), g = Dc gx on F.
+  for (int j = threadIdx.x; j < n; j += NT) n0[j] = mc[j] * it.dc[j] * W.gx[d.col0 + j];
+  __syncthreads();
+  double rel = 0.0;
+  // z = lambda (m0), r = n0, q = n1, p = m1, s = m2; B products: q = T (p), s = A (r).
+  int itx = cgls<GEO>(T64, A64, mc, mr, n, m, m0, n0, n1, m1, m2, tol, maxit, lds, red, &bcast,
+                      &rel);
+  for (int i = threadIdx.x; i < m; i += NT) gb[i] = it.dr[i] * m0[i];
+  if (threadIdx.x == 0) {
+    W.iters[2 * lp] = itx;
+    W.resid[2 * lp] = rel;
+  }
+  __syncthreads();
+  // y side: B = M (rows: R rows, cols: F columns), g = Dr gy on R.
+  for (int i = threadIdx.x; i < m; i += NT) m0[i] = mr[i] * it.dr[i] * W.gy[d.row0 + i];
+  __syncthreads();
+  // z = mu (n0), r = m0, q = m1, p = n1, s = n2; B products: q = A (p), s = T (r).
+  const int ity = cgls<GEO>(A64, T64, mr, mc, m, n, n0, m0, m1, n1, n2, tol, maxit, lds, red,
+                            &bcast, &rel);
+  for (int j = threadIdx.x; j < n; j += NT) gc[j] = -it.dc[j] * n0[j];
+  if (threadIdx.x == 0) {
+    W.iters[2 * lp + 1] = ity;
+    W.resid[2 * lp + 1] = rel;
+  }
+}
+
 // Report (src/pdhg/pdhg.cpp:486-501) for each LP: evaluate its best point on FP64 values.
 template <class GEO, int G, typename X>
 __global__ __launch_bounds__(GEO::kThreads) void k_batch_finish(Args<double, X> B,
@@ -701,6 +893,10 @@ struct pdhg_batch {
   bool any_t_long = false;
   double *part_p = nullptr, *part_d = nullptr, *part_er = nullptr, *part_ec = nullptr,
          *lr_p = nullptr;
+  // Adjoint work space (allocated on first use).
+  double *adj_mc = nullptr, *adj_mr = nullptr, *adj_gx = nullptr, *adj_gy = nullptr,
+         *adj_gb = nullptr, *adj_gc = nullptr, *adj_n[3] = {}, *adj_m[3] = {}, *adj_res = nullptr;
+  int32_t* adj_it = nullptr;
   hipGraphExec_t exec = nullptr;
   int chunk = kDefaultChunkAttempts;
   bool uploaded = false, started = false;
@@ -1439,6 +1635,56 @@ int pdhg_batch_finish(pdhg_batch* h, double* col_value, double* y, double* reduc
   if (residuals)
     for (int32_t k = 0; k < h->nlp; ++k)
       std::memcpy(residuals + 9 * static_cast<size_t>(k), h->ctl_host[k].final_res, sizeof(double) * 9);
+  return PDHG_OK;
+}
+
+int pdhg_batch_adjoint(pdhg_batch* h, const double* grad_x, const double* grad_y, double tol,
+                       int max_iterations, double* grad_b, double* grad_c, int32_t* iterations,
+                       double* residuals) {
+  int rc = check(h, true);
+  if (rc) return rc;
+  if (!grad_x || !grad_y || !grad_b || !grad_c || !(tol > 0.0) || max_iterations < 0)
+    return bfail(h, PDHG_E_ARGUMENT, "bad adjoint arguments");
+  BHIP_TRY(h, hipSetDevice(h->device));
+  const size_t un = static_cast<size_t>(h->sum_n), um = static_cast<size_t>(h->sum_m);
+  if (!h->adj_mc) {
+    if ((rc = balloc(h, &h->adj_mc, un)) || (rc = balloc(h, &h->adj_mr, um)) ||
+        (rc = balloc(h, &h->adj_gx, un)) || (rc = balloc(h, &h->adj_gy, um)) ||
+        (rc = balloc(h, &h->adj_gb, um)) || (rc = balloc(h, &h->adj_gc, un)) ||
+        (rc = balloc(h, &h->adj_res, 2 * static_cast<size_t>(h->nlp))) ||
+        (rc = balloc(h, &h->adj_it, 2 * static_cast<size_t>(h->nlp))))
+      return rc;
+    for (int k = 0; k < 3; ++k)
+      if ((rc = balloc(h, &h->adj_n[k], un)) || (rc = balloc(h, &h->adj_m[k], um))) return rc;
+  }
+  BHIP_TRY(h, hipMemcpyAsync(h->adj_gx, grad_x, sizeof(double) * un, hipMemcpyHostToDevice, h->stream));
+  BHIP_TRY(h, hipMemcpyAsync(h->adj_gy, grad_y, sizeof(double) * um, hipMemcpyHostToDevice, h->stream));
+  AdjWork w;
+  w.mask_c = h->adj_mc;
+  w.mask_r = h->adj_mr;
+  w.gx = h->adj_gx;
+  w.gy = h->adj_gy;
+  w.gb = h->adj_gb;
+  w.gc = h->adj_gc;
+  w.vn0 = h->adj_n[0];
+  w.vn1 = h->adj_n[1];
+  w.vn2 = h->adj_n[2];
+  w.vm0 = h->adj_m[0];
+  w.vm1 = h->adj_m[1];
+  w.vm2 = h->adj_m[2];
+  w.iters = h->adj_it;
+  w.resid = h->adj_res;
+  Args<double, double> a = args_of<double, double>(h, 1);
+  a.order = h->order_all;
+  k_batch_adjoint<Geo1, 1><<<h->nlp, Geo1::kThreads, 0, h->stream>>>(a, w, tol, max_iterations);
+  BHIP_TRY(h, hipGetLastError());
+  BHIP_TRY(h, hipMemcpyAsync(grad_b, h->adj_gb, sizeof(double) * um, hipMemcpyDeviceToHost, h->stream));
+  BHIP_TRY(h, hipMemcpyAsync(grad_c, h->adj_gc, sizeof(double) * un, hipMemcpyDeviceToHost, h->stream));
+  if (iterations)
+    BHIP_TRY(h, hipMemcpyAsync(iterations, h->adj_it, sizeof(int32_t) * 2 * h->nlp, hipMemcpyDeviceToHost, h->stream));
+  if (residuals)
+    BHIP_TRY(h, hipMemcpyAsync(residuals, h->adj_res, sizeof(double) * 2 * h->nlp, hipMemcpyDeviceToHost, h->stream));
+  BHIP_TRY(h, hipStreamSynchronize(h->stream));
   return PDHG_OK;
 }
 

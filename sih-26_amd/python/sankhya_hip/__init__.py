@@ -99,6 +99,14 @@ def load_library() -> C.CDLL:
     lib.sankhya_pdhg_solve_batch.restype = C.c_int
     lib.sankhya_pdhg_solve_batch.argtypes = [C.POINTER(vp), C.c_int32, C.c_char_p, vp, vp, vp, vp,
                                              C.POINTER(Result)]
+    lib.sankhya_batch_solve.restype = vp
+    lib.sankhya_batch_solve.argtypes = [C.POINTER(vp), C.c_int32, C.c_char_p, vp, vp, vp, vp,
+                                        C.POINTER(Result)]
+    lib.sankhya_batch_adjoint.restype = C.c_int
+    lib.sankhya_batch_adjoint.argtypes = [vp, _f64p, _f64p, C.c_double, C.c_int, _f64p, _f64p,
+                                          np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
+                                          _f64p]
+    lib.sankhya_batch_free.argtypes = [vp]
     lib.sankhya_pdhg_scaling.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
                                          _f64p, _f64p]
     lib.sankhya_pdhg_device_open.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
@@ -262,6 +270,65 @@ def _solution_of(res, x, y, d, act, stats_extra=None) -> Solution:
                     x, y, d, act, res.algorithm.decode(), res.message.decode(),
                     res.solve_seconds, res.primal_infeasibility, res.dual_infeasibility,
                     res.complementarity_violation, res.relative_gap, stats)
+
+
+class BatchSession:
+    """A solved batch kept on the device (sankhya_batch_solve) so that vector-Jacobian products
+    of the LP solution maps can be taken afterwards (sankhya_batch_adjoint)."""
+
+    def __init__(self, models, **options):
+        self._lib = load_library()
+        self.models = list(models)
+        nlp = len(self.models)
+        self.dims = [md.dims for md in self.models]
+        sn = sum(d[1] for d in self.dims)
+        sm = sum(d[0] for d in self.dims)
+        x, d, y, act = np.zeros(max(sn, 1)), np.zeros(max(sn, 1)), np.zeros(max(sm, 1)), np.zeros(max(sm, 1))
+        res = (Result * max(nlp, 1))()
+        handles = (C.c_void_p * max(nlp, 1))(*[md._h.value for md in self.models])
+        h = self._lib.sankhya_batch_solve(handles, nlp, _options_text(options), x.ctypes.data,
+                                          y.ctypes.data, d.ctypes.data, act.ctypes.data, res)
+        if not h:
+            raise ValueError(res[0].message.decode())
+        self._h = C.c_void_p(h)
+        self.solutions, on, om = [], 0, 0
+        for k, (m, n, _) in enumerate(self.dims):
+            self.solutions.append(_solution_of(res[k], x[on:on + n].copy(), y[om:om + m].copy(),
+                                               d[on:on + n].copy(), act[om:om + m].copy()))
+            on += n
+            om += m
+
+    def adjoint(self, grad_col_value, grad_row_dual, tol=1e-14, max_iterations=50000):
+        """dL/dc and dL/d(active row bound) per LP for upstream dL/dx and dL/d(row_dual) (lists
+        of arrays, one per LP, or None for zero). Returns (grad_c, grad_b, iterations, residuals)."""
+        def cat(parts, idx):
+            out = []
+            for k, dm in enumerate(self.dims):
+                v = None if parts is None else parts[k]
+                out.append(np.zeros(dm[idx]) if v is None else np.asarray(v, dtype=np.float64).reshape(dm[idx]))
+            return np.ascontiguousarray(np.concatenate(out) if out else np.zeros(1))
+        gx, gy = cat(grad_col_value, 1), cat(grad_row_dual, 0)
+        gc, gb = np.zeros(max(gx.size, 1)), np.zeros(max(gy.size, 1))
+        its = np.zeros(2 * max(len(self.dims), 1), dtype=np.int32)
+        res = np.zeros(2 * max(len(self.dims), 1))
+        rc = self._lib.sankhya_batch_adjoint(self._h, gx, gy, float(tol), int(max_iterations), gc, gb, its, res)
+        if rc != 0:
+            raise RuntimeError(f"adjoint failed with code {rc}")
+        out_c, out_b, on, om = [], [], 0, 0
+        for m, n, _ in self.dims:
+            out_c.append(gc[on:on + n].copy())
+            out_b.append(gb[om:om + m].copy())
+            on += n
+            om += m
+        return out_c, out_b, its.reshape(-1, 2)[: len(self.dims)], res.reshape(-1, 2)[: len(self.dims)]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sankhya_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def solve_batch(models, **options):

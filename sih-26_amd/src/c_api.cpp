@@ -4,6 +4,8 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <memory>
+#include <vector>
 #include <string>
 
 #include "../../include/sankhya_pdhg.h"
@@ -185,9 +187,14 @@ int sankhya_pdhg_solve(const sankhya_model* h, const char* options, double* col_
   return 0;
 }
 
-int sankhya_pdhg_solve_batch(const sankhya_model* const* models, int32_t nlp, const char* options,
-                             double* col_value, double* row_dual, double* col_dual,
-                             double* row_activity, sankhya_result* results) {
+struct sankhya_batch {
+  std::unique_ptr<hip_engine::BatchSession> session;
+};
+
+sankhya_batch* sankhya_batch_solve(const sankhya_model* const* models, int32_t nlp,
+                                   const char* options, double* col_value, double* row_dual,
+                                   double* col_dual, double* row_activity,
+                                   sankhya_result* results) {
   Options o;
   o.set_bool("log_to_console", false);
   std::string error;
@@ -197,12 +204,24 @@ int sankhya_pdhg_solve_batch(const sankhya_model* const* models, int32_t nlp, co
       results->status = static_cast<int32_t>(SolveStatus::kModelError);
       copy_text(results->message, sizeof(results->message), error.empty() ? "bad batch" : error);
     }
-    return -1;
+    return nullptr;
   }
   std::vector<const Model*> list(static_cast<std::size_t>(nlp));
   for (int32_t k = 0; k < nlp; ++k) list[k] = models[k] ? &models[k]->model : nullptr;
   Logger logger(o.get_bool("log_to_console") ? stdout : nullptr, LogLevel::kInfo);
-  const std::vector<Solution> sols = gpu::solve_pdhg_batch(list, o, logger, nullptr);
+  auto* out_session = new sankhya_batch();
+  try {
+    out_session->session = hip_engine::open_batch(list, o, logger, nullptr);
+  } catch (const std::exception& e) {
+    out_session->session = std::make_unique<hip_engine::BatchSession>();
+    out_session->session->models = list;
+    out_session->session->solutions.assign(list.size(), Solution{});
+    for (Solution& s : out_session->session->solutions) {
+      s.status = SolveStatus::kNumericalError;
+      s.message = std::string("batched PDHG failed: ") + e.what();
+    }
+  }
+  const std::vector<Solution>& sols = out_session->session->solutions;
   const auto& stats = hip_engine::last_batch_stats();
   std::size_t on = 0, om = 0;
   for (int32_t k = 0; k < nlp; ++k) {
@@ -246,6 +265,29 @@ int sankhya_pdhg_solve_batch(const sankhya_model* const* models, int32_t nlp, co
     copy_text(out->algorithm, sizeof(out->algorithm), s.algorithm);
     copy_text(out->message, sizeof(out->message), s.message);
   }
+  return out_session;
+}
+
+int sankhya_batch_adjoint(const sankhya_batch* session, const double* grad_col_value,
+                          const double* grad_row_dual, double tol, int max_iterations,
+                          double* grad_cost, double* grad_row_bound, int32_t* iterations,
+                          double* residuals) {
+  if (!session || !session->session || !grad_col_value || !grad_row_dual || !grad_cost ||
+      !grad_row_bound)
+    return PDHG_E_ARGUMENT;
+  return session->session->adjoint(grad_col_value, grad_row_dual, tol, max_iterations, grad_cost,
+                                   grad_row_bound, iterations, residuals);
+}
+
+void sankhya_batch_free(sankhya_batch* session) { delete session; }
+
+int sankhya_pdhg_solve_batch(const sankhya_model* const* models, int32_t nlp, const char* options,
+                             double* col_value, double* row_dual, double* col_dual,
+                             double* row_activity, sankhya_result* results) {
+  sankhya_batch* s = sankhya_batch_solve(models, nlp, options, col_value, row_dual, col_dual,
+                                         row_activity, results);
+  if (!s) return -1;
+  sankhya_batch_free(s);
   return 0;
 }
 

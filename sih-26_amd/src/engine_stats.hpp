@@ -3,8 +3,10 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
+#include "../../include/pdhg_batch.h"
 #include "sankhya/logging.hpp"
 #include "sankhya/model.hpp"
 #include "sankhya/options.hpp"
@@ -26,5 +28,21 @@ const std::vector<EngineStats>& last_batch_stats();
 Solution run(const Model& model, const Options& options, Logger& logger, SolveControl* control);
 std::vector<Solution> run_batch(const std::vector<const Model*>& models, const Options& options,
                                 Logger& logger, SolveControl* control);
+
+// A solved batch that keeps its device state for vector-Jacobian products.
+struct BatchSession {
+  std::vector<const Model*> models;
+  std::vector<std::size_t> live;  // models on the device, in device order
+  std::vector<Solution> solutions;
+  ::pdhg_batch* handle = nullptr;
+  ~BatchSession();
+  // dL/dc and dL/d(active row bound) for upstream dL/dx and dL/d(row_dual), concatenated in
+  // model order (user-facing signs: the model's own sense and row duals).
+  int adjoint(const double* grad_x, const double* grad_row_dual, double tol, int max_iterations,
+              double* grad_c, double* grad_bound, int32_t* iterations, double* residuals) const;
+};
+std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models,
+                                         const Options& options, Logger& logger,
+                                         SolveControl* control);
 
 }  // namespace sankhya::hip_engine

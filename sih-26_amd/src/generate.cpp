@@ -29,13 +29,16 @@ struct Rng {
   std::uint64_t s;
   bool has_spare = false;
   double spare = 0.0;
-  explicit Rng(std::uint64_t seed) : s(seed * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL) {}
-  std::uint64_t next() {
-    std::uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  // The seed goes through the output mix before it becomes the state: the state advances by
+  // the increment below, so seeding with an affine function of the seed would make seed k + 1
+  // replay seed k's stream one draw later.
+  explicit Rng(std::uint64_t seed) : s(mix(seed ^ 0x632BE59BD9B4E019ULL)) {}
+  static std::uint64_t mix(std::uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
   }
+  std::uint64_t next() { return mix(s += 0x9E3779B97F4A7C15ULL); }
   double uniform() { return static_cast<double>(next() >> 11) * 0x1.0p-53; }  // [0,1)
   double normal() {
     if (has_spare) {

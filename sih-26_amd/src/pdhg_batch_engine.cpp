@@ -8,6 +8,7 @@
 // optionally the schedule tuner, then graph chunks until every LP has stopped or the time
 // limit / interrupt ends the batch.
 #include <algorithm>
+#include <memory>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -26,18 +27,19 @@ const std::vector<EngineStats>& last_batch_stats() { return g_batch_stats; }
 
 namespace {
 
-struct BatchHandle {
-  pdhg_batch* h = nullptr;
-  ~BatchHandle() { pdhg_batch_destroy(h); }
-};
-
 }  // namespace
 
-std::vector<Solution> run_batch(const std::vector<const Model*>& models, const Options& options,
-                                Logger& logger, SolveControl* control) {
+BatchSession::~BatchSession() { pdhg_batch_destroy(handle); }
+
+std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models,
+                                         const Options& options, Logger& logger,
+                                         SolveControl* control) {
+  auto session = std::make_unique<BatchSession>();
+  session->models = models;
   Timer timer;
   const std::size_t nlp = models.size();
-  std::vector<Solution> out(nlp);
+  std::vector<Solution>& out = session->solutions;
+  out.assign(nlp, Solution{});
   g_batch_stats.assign(nlp, EngineStats{});
   const bool mixed = options.get_string("pdhg_precision") == "mixed";
   const std::string algorithm = mixed ? "pdhg-gpu-batch-mixed" : "pdhg-gpu-batch";
@@ -59,7 +61,7 @@ std::vector<Solution> run_batch(const std::vector<const Model*>& models, const O
     }
     live.push_back(k);
   }
-  if (live.empty()) return out;
+  if (live.empty()) return session;
   auto fail_all = [&](const std::string& what) {
     for (const std::size_t k : live) {
       out[k].status = SolveStatus::kNumericalError;
@@ -67,7 +69,9 @@ std::vector<Solution> run_batch(const std::vector<const Model*>& models, const O
       out[k].solve_seconds = timer.elapsed_seconds();
     }
     logger.error("MI355X batched PDHG: {}", what);
-    return out;
+    pdhg_batch_destroy(session->handle);
+    session->handle = nullptr;
+    return std::move(session);
   };
   const int device = engine_device(options, logger);
   char desc[256] = {0};
@@ -86,14 +90,15 @@ std::vector<Solution> run_batch(const std::vector<const Model*>& models, const O
   const std::vector<int32_t>& n = in.n;
   const double setup_seconds = timer.elapsed_seconds();
 
-  BatchHandle dev;
-  int rc = pdhg_batch_create(device, mixed ? PDHG_PRECISION_MIXED : PDHG_PRECISION_FP64, &dev.h);
+  session->live = live;
+  pdhg_batch*& dev_h = session->handle;
+  int rc = pdhg_batch_create(device, mixed ? PDHG_PRECISION_MIXED : PDHG_PRECISION_FP64, &dev_h);
   if (rc != PDHG_OK) return fail_all("batch handle creation failed (code " + std::to_string(rc) + ")");
   auto dev_error = [&](int code, const char* step) {
     return fail_all(std::string(step) + " failed (code " + std::to_string(code) + "): " +
-                    pdhg_batch_last_error(dev.h));
+                    pdhg_batch_last_error(dev_h));
   };
-  rc = pdhg_batch_upload(dev.h, static_cast<int32_t>(nb), in.m.data(), in.n.data(), in.nnz.data(),
+  rc = pdhg_batch_upload(dev_h, static_cast<int32_t>(nb), in.m.data(), in.n.data(), in.nnz.data(),
                          in.a_rp.data(), in.a_ci.data(), in.a_v.data(), in.t_rp.data(),
                          in.t_ci.data(), in.t_v.data(), in.chat.data(), in.lhat.data(),
                          in.uhat.data(), in.rlhat.data(), in.ruhat.data(), in.dc.data(),
@@ -101,7 +106,7 @@ std::vector<Solution> run_batch(const std::vector<const Model*>& models, const O
                          in.ru.data(), in.bound_norm.data(), in.cost_norm.data());
   if (rc != PDHG_OK) return dev_error(rc, "upload");
   std::vector<double> norms(nb, 1.0);
-  rc = pdhg_batch_spectral_norms(dev.h, in.v0.data(), kPowerIterations, norms.data());
+  rc = pdhg_batch_spectral_norms(dev_h, in.v0.data(), kPowerIterations, norms.data());
   if (rc != PDHG_OK) return dev_error(rc, "spectral norms");
 
   const double tolerance = options.get_double("pdhg_tolerance");
@@ -121,7 +126,7 @@ std::vector<Solution> run_batch(const std::vector<const Model*>& models, const O
               mixed ? "mixed" : "fp64");
   if (options.get_bool("pdhg_batch_tune")) {
     std::vector<int32_t> geo(nb, 0);
-    rc = pdhg_batch_tune(dev.h, &params, norms.data(),
+    rc = pdhg_batch_tune(dev_h, &params, norms.data(),
                          static_cast<int>(options.get_int("pdhg_batch_tune_attempts")), geo.data());
     if (rc != PDHG_OK) return dev_error(rc, "schedule tuning");
     int count[3] = {0, 0, 0}, coop = 0;
@@ -131,7 +136,7 @@ std::vector<Solution> run_batch(const std::vector<const Model*>& models, const O
     }
     logger.info("Schedule: {} cooperative, {} persistent at 256 threads, {} at 512, {} at 1024", coop, count[0], count[1], count[2]);
   }
-  rc = pdhg_batch_start(dev.h, &params, norms.data());
+  rc = pdhg_batch_start(dev_h, &params, norms.data());
   if (rc != PDHG_OK) return dev_error(rc, "start");
   const double loop_start = timer.elapsed_seconds();
   SolveStatus stop_status = SolveStatus::kIterationLimit;
@@ -145,7 +150,7 @@ std::vector<Solution> run_batch(const std::vector<const Model*>& models, const O
       stop_status = SolveStatus::kInterrupted;
       break;
     }
-    rc = pdhg_batch_run(dev.h, 1, &running);
+    rc = pdhg_batch_run(dev_h, 1, &running);
     if (rc != PDHG_OK) return dev_error(rc, "iteration");
   }
   const double loop_seconds = timer.elapsed_seconds() - loop_start;
@@ -159,9 +164,9 @@ std::vector<Solution> run_batch(const std::vector<const Model*>& models, const O
   std::vector<double> x(static_cast<std::size_t>(sum_n)), y(static_cast<std::size_t>(sum_m)),
       red(static_cast<std::size_t>(sum_n)), res(9 * nb);
   std::vector<pdhg_state> states(nb);
-  rc = pdhg_batch_finish(dev.h, x.data(), y.data(), red.data(), res.data());
+  rc = pdhg_batch_finish(dev_h, x.data(), y.data(), red.data(), res.data());
   if (rc != PDHG_OK) return dev_error(rc, "report");
-  rc = pdhg_batch_states(dev.h, states.data());
+  rc = pdhg_batch_states(dev_h, states.data());
   if (rc != PDHG_OK) return dev_error(rc, "state read");
   const double total = timer.elapsed_seconds();
   std::int64_t off_n = 0, off_m = 0;
@@ -178,7 +183,7 @@ std::vector<Solution> run_batch(const std::vector<const Model*>& models, const O
     es.attempts = st.attempts;
     es.restarts = st.restarts;
     es.ticks = st.ticks;
-    es.device_bytes = pdhg_batch_bytes(dev.h);
+    es.device_bytes = pdhg_batch_bytes(dev_h);
     es.spectral_norm = norms[q];
     es.setup_seconds = setup_seconds;
     es.loop_seconds = loop_seconds;
@@ -195,7 +200,60 @@ std::vector<Solution> run_batch(const std::vector<const Model*>& models, const O
   }
   logger.info("Batch of {} LPs: setup {:.3f}s, iterations {:.3f}s, total {:.3f}s", nb, setup_seconds,
               loop_seconds, total);
-  return out;
+  return session;
+}
+
+std::vector<Solution> run_batch(const std::vector<const Model*>& models, const Options& options,
+                                Logger& logger, SolveControl* control) {
+  return open_batch(models, options, logger, control)->solutions;
+}
+
+int BatchSession::adjoint(const double* grad_x, const double* grad_row_dual, double tol,
+                          int max_iterations, double* grad_c, double* grad_bound,
+                          int32_t* iterations, double* residuals) const {
+  // Model order -> device order (LPs that failed validation are not on the device).
+  std::vector<std::int64_t> off_n(models.size() + 1, 0), off_m(models.size() + 1, 0);
+  for (std::size_t k = 0; k < models.size(); ++k) {
+    off_n[k + 1] = off_n[k] + (models[k] ? models[k]->num_cols() : 0);
+    off_m[k + 1] = off_m[k] + (models[k] ? models[k]->num_rows() : 0);
+  }
+  std::fill(grad_c, grad_c + off_n.back(), 0.0);
+  std::fill(grad_bound, grad_bound + off_m.back(), 0.0);
+  if (iterations) std::fill(iterations, iterations + 2 * models.size(), 0);
+  if (residuals) std::fill(residuals, residuals + 2 * models.size(), 0.0);
+  if (!handle || live.empty()) return PDHG_E_STATE;
+  std::vector<double> gx, gy;
+  for (const std::size_t k : live) {
+    const double sense = models[k]->sense_multiplier();
+    gx.insert(gx.end(), grad_x + off_n[k], grad_x + off_n[k + 1]);
+    // row_dual = sense * (-y)  =>  dL/dy = -sense * dL/d(row_dual)
+    for (std::int64_t i = off_m[k]; i < off_m[k + 1]; ++i) gy.push_back(-sense * grad_row_dual[i]);
+  }
+  std::vector<double> gb(gy.size()), gc(gx.size()), res(2 * live.size());
+  std::vector<int32_t> its(2 * live.size());
+  const int rc = pdhg_batch_adjoint(handle, gx.data(), gy.data(), tol, max_iterations, gb.data(),
+                                    gc.data(), its.data(), res.data());
+  if (rc != PDHG_OK) return rc;
+  std::size_t pn = 0, pm = 0;
+  for (std::size_t q = 0; q < live.size(); ++q) {
+    const std::size_t k = live[q];
+    const double sense = models[k]->sense_multiplier();
+    const std::int64_t n = off_n[k + 1] - off_n[k], m = off_m[k + 1] - off_m[k];
+    // The engine's cost is sense * c.
+    for (std::int64_t j = 0; j < n; ++j) grad_c[off_n[k] + j] = sense * gc[pn + j];
+    for (std::int64_t i = 0; i < m; ++i) grad_bound[off_m[k] + i] = gb[pm + i];
+    if (iterations) {
+      iterations[2 * k] = its[2 * q];
+      iterations[2 * k + 1] = its[2 * q + 1];
+    }
+    if (residuals) {
+      residuals[2 * k] = res[2 * q];
+      residuals[2 * k + 1] = res[2 * q + 1];
+    }
+    pn += static_cast<std::size_t>(n);
+    pm += static_cast<std::size_t>(m);
+  }
+  return PDHG_OK;
 }
 
 }  // namespace hip_engine

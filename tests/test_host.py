@@ -74,6 +74,13 @@ def test_generators_are_deterministic_and_shaped():
         assert np.array_equal(d1[k], d2[k])
     _, d3 = cm.c0(5)
     assert not np.array_equal(d1["val"], d3["val"])
+    # consecutive seeds give unrelated instances (no shifted replay of one random stream)
+    for k in ("cost", "row_upper", "val"):
+        a, b = (d1[k], d3[k]) if k != "val" else (np.sort(d1[k]), np.sort(d3[k]))
+        assert np.mean(np.isclose(a, b)) < 0.01, k
+    _, e1 = cm.generated("blend", 50, 120, 5, 5000)
+    _, e2 = cm.generated("blend", 50, 120, 5, 5001)
+    assert np.mean(np.isclose(e1["cost"], e2["cost"])) < 0.05
     # C0: 5 distinct rows per column, all equality rows, box [0, 10]
     assert d1["nnz"] == 5 * 3000 and np.all(np.diff(d1["colptr"]) == 5)
     for j in range(0, 3000, 97):
