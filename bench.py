@@ -13,6 +13,15 @@ with |b| in the hundreds on the dense rows) is not reached within 1e6 iterations
 A step is one whole solve: host preconditioning, upload, power iteration, the iterations and
 the report; instance generation is outside the timed region.
 
+Beside the headline line's `value`, two measurements for the metric's "mixed/batched vs plain
+FP64" half:
+  - `mixed`: the same LP solved once with FP32 iterate deltas and FP64 ticks (LPs/s, iterations,
+    objective difference from the FP64 solve);
+  - `batched`: 512 LPs of the batched workload (m log-uniform in [200, 20000], n = 2.5 m, 6
+    nonzeros per column, relative KKT <= 1e-6) solved in one batch in mixed precision, against
+    plain FP64 single-LP solves of a 16-LP sample of the same LPs (their LPs/s extrapolated to
+    the batch); `speedup` is the ratio of the two rates.
+
 Prints one JSON line. One GPU only: --gpus above 1 exits with a message.
 """
 from __future__ import annotations
@@ -94,6 +103,39 @@ def cpu_baseline(arrays, gpu_iterations, tolerance, budget_s=20.0):
                       f"({est:.1f}s per LP); status of the sample run {r0['status']}"}
 
 
+def batched_comparison(sh, count):
+    """Mixed-precision batched solve of `count` LPs of the batched workload, against FP64
+    single-LP solves of a 16-LP sample (every count/16-th LP)."""
+    import sankhya_hip.workloads as wl
+
+    opts = dict(pdhg_tolerance=1e-6, pdhg_stop_rule="relative_kkt", iteration_limit=1_000_000)
+    t = time.perf_counter()
+    models = wl.batched(count)
+    log(f"[bench] batched workload: {count} LPs, nnz {sum(m.dims[2] for m in models)} "
+        f"(generated in {time.perf_counter()-t:.1f}s)")
+    t = time.perf_counter()
+    sols = sh.solve_batch(models, pdhg_precision="mixed", pdhg_batch_tune=False, **opts)
+    tb = time.perf_counter() - t
+    solved = sum(1 for s in sols if s.status in ("optimal", "feasible"))
+    log(f"[bench] batched mixed: {solved}/{count} solved in {tb:.2f}s")
+    sample = models[:: max(1, count // 16)]
+    t = time.perf_counter()
+    single = [sh.solve(m, **opts) for m in sample]
+    ts = time.perf_counter() - t
+    single_ok = sum(1 for s in single if s.status in ("optimal", "feasible"))
+    single_rate = single_ok / ts if ts > 0 else 0.0
+    log(f"[bench] fp64 single-LP sample: {single_ok}/{len(sample)} in {ts:.2f}s")
+    its = sorted(s.iterations for s in sols)
+    return {"workload": f"batched-{count}: m log-uniform [200, 20000], n = 2.5 m, 6 nnz/col, "
+                        "relative KKT <= 1e-6",
+            "value": solved / tb, "unit": "LPs/s", "precision": "mixed", "seconds": tb,
+            "solved": solved, "lps": count, "max_iterations": its[-1],
+            "median_iterations": its[len(its) // 2],
+            "fp64_single_value": single_rate, "fp64_single_sample": len(sample),
+            "fp64_single_seconds": ts,
+            "speedup": (solved / tb) / single_rate if single_rate > 0 else None}
+
+
 def host_info():
     try:
         model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
@@ -113,6 +155,9 @@ def main():
     ap.add_argument("--timing-reps", type=int, default=50)
     ap.add_argument("--iteration-limit", type=int, default=1_000_000,
                     help="cap per solve (profiling runs); the default is the engine's own ceiling")
+    ap.add_argument("--batched-lps", type=int, default=512,
+                    help="LPs in the batched comparison (0 skips it)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -171,6 +216,20 @@ def main():
     dom = kd if d_ms >= p_ms else kp
     log(f"[bench] k_primal {p_ms*1e3:.1f} us {kp['gbs']:.0f} GB/s; k_dual {d_ms*1e3:.1f} us {kd['gbs']:.0f} GB/s")
 
+    mixed = None
+    if not args.no_mixed and args.precision == "fp64":
+        ts = time.perf_counter()
+        sm = sh.solve(model, **dict(opts, pdhg_precision="mixed"))
+        tm = time.perf_counter() - ts
+        mixed = {"value": (1.0 / tm) if sm.status in ("optimal", "feasible") else 0.0,
+                 "unit": "LPs/s", "status": sm.status, "iterations": sm.iterations,
+                 "fp32_iterations": sm.stats["fp32_iterations"], "seconds": tm,
+                 "objective_rel_diff_vs_fp64": abs(sm.objective - last.objective) / max(1.0, abs(last.objective))}
+        log(f"[bench] mixed: {sm.status} it={sm.iterations} t={tm:.3f}s")
+    batched = None
+    if args.batched_lps > 0:
+        batched = batched_comparison(sh, args.batched_lps)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline (oracle, one core) on a bounded sample")
@@ -201,6 +260,8 @@ def main():
                      "kernel_us": dom["ms"] * 1e3, "bytes_per_launch": dom["bytes"],
                      "other_kernel": {k: v for k, v in (kp if dom is kd else kd).items()}},
         "cpu_baseline": cpu,
+        "mixed": mixed,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

@@ -10,6 +10,7 @@ import pytest
 
 import oracle as orc
 import sankhya_hip as sh
+import sankhya_hip.workloads as wl
 from mps import read_mps
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -89,13 +90,10 @@ KNOWN_ANSWERS = [
 
 
 def c3_sizes(count: int, seed: int = 1000):
-    """Row counts of the batched workload: m log-uniform in [200, 20000] (rounded), n = 2.5 m."""
-    rng = np.random.default_rng(seed)
-    m = np.rint(np.exp(rng.uniform(np.log(200.0), np.log(20000.0), count))).astype(int)
-    return [(int(mi), int(round(2.5 * mi))) for mi in m]
+    """Row counts of the batched workload (sankhya_hip.workloads.batched_sizes)."""
+    return wl.batched_sizes(count, seed)
 
 
 def c3_batch(count: int, seed: int = 1000):
-    """`count` LPs of the batched workload: the 1,000 x 3,000 instance's generator (uniform rows,
-    equality constraints, 6 nonzeros per column) at c3_sizes, LP i seeded seed + i."""
-    return [sh.Model.generate("uniform", m, n, 6, seed + i) for i, (m, n) in enumerate(c3_sizes(count, seed))]
+    """`count` LPs of the batched workload (sankhya_hip.workloads.batched)."""
+    return wl.batched(count, seed)

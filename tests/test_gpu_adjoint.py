@@ -27,6 +27,7 @@ import scipy.sparse.linalg as spla
 
 import common as cm
 import sankhya_hip as sh
+import sankhya_hip.workloads as wl
 
 torch = pytest.importorskip("torch")
 from sankhya_hip.torch_layer import LPFamily, lp_layer  # noqa: E402
@@ -44,7 +45,7 @@ def device():
 
 
 def blend(count, m=5000, n=12000, seed0=5000):
-    return [sh.Model.generate("blend", m, n, 5, seed0 + i) for i in range(count)]
+    return wl.blend(count, m, n, seed0)
 
 
 def active_maps(d, sol):

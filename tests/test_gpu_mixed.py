@@ -75,13 +75,13 @@ def test_promote_continues_the_same_run():
     mdl, _ = cm.c0(2)
     norm = sh.solve(mdl, iteration_limit=1).stats["spectral_norm"]
     dev = sh.Device(mdl, precision="mixed")
-    dev.start(tolerance=1e-4, spectral_norm=norm, stop_rule=1, iteration_limit=200000)
+    dev.start(tolerance=1e-4, spectral_norm=norm, stop_rule=1, iteration_limit=1000000)
     st = dev.run(1)
     while not (st.done or st.limit_hit):
         st = dev.run(8)
     assert st.done
     it0, r0 = st.iteration, st.restarts
-    dev.promote(tolerance=1e-7, spectral_norm=norm, stop_rule=1, iteration_limit=200000)
+    dev.promote(tolerance=1e-7, spectral_norm=norm, stop_rule=1, iteration_limit=1000000)
     st = dev.run(1)
     assert st.iteration >= it0 and st.restarts >= r0
     while not (st.done or st.limit_hit):
