@@ -83,13 +83,25 @@ inline HostPlan build_plan(int32_t rows, int32_t ncols, const int32_t* rowptr, c
     stream.push_back(item);
     logical->push_back(item);
   }
-  // Dispatch order: position p runs on XCD p % kXcds; take that band's next segment, else a
+  // Dispatch order: stream items spread evenly through the segments (a stream item is placed
+  // whenever the share of stream items placed so far is at most the share of segments), so the
+  // latency-bound short rows run beside the bandwidth-bound long-row segments instead of after
+  // them. A segment goes to position p on XCD p % kXcds: that band's next segment, else a
   // stream item, else the fullest band's.
   std::vector<size_t> next(kXcds, 0);
   size_t ns = 0;
   const size_t total = logical->size();
+  const size_t nsegs = total - stream.size();
+  size_t segs_placed = 0;
   for (size_t p = 0; p < total; ++p) {
     const int x = static_cast<int>(p % kXcds);
+    if (ns < stream.size() &&
+        (nsegs == 0 || static_cast<double>(ns) * static_cast<double>(nsegs) <=
+                           static_cast<double>(segs_placed) * static_cast<double>(stream.size()))) {
+      hp.items.push_back(stream[ns++]);
+      continue;
+    }
+    ++segs_placed;
     if (next[x] < band[x].size()) {
       hp.items.push_back(band[x][next[x]++]);
     } else if (ns < stream.size()) {
