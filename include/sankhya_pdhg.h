@@ -43,6 +43,12 @@ void sankhya_model_export(const sankhya_model* model, int32_t* colptr, int32_t* 
 
 void sankhya_model_free(sankhya_model* model);
 
+/* Model and row/column names for solution files (the reference Model's name, source_path,
+ * col_names, row_names; include/sankhya/model.hpp). Any pointer may be NULL (left unchanged);
+ * col_names holds n entries and row_names m, a NULL entry meaning the default C<j> / R<i>. */
+int sankhya_model_set_names(sankhya_model* model, const char* name, const char* source_path,
+                            const char* const* col_names, const char* const* row_names);
+
 typedef struct {
   int32_t status; /* sankhya::SolveStatus */
   int64_t iterations, attempts, restarts, ticks, device_bytes;
@@ -91,6 +97,16 @@ int sankhya_batch_adjoint(const sankhya_batch* session, const double* grad_col_v
                           double* residuals);
 
 void sankhya_batch_free(sankhya_batch* session);
+
+/* Write the solution file, the engine's certificate output (replaces io::write_solution,
+ * include/sankhya/io.hpp:65-72, format of src/io/writer.cpp:121-284), from a result and its
+ * vectors as sankhya_pdhg_solve returned them (NULL vectors are written as zeros). options:
+ * as for sankhya_pdhg_solve; mip_relative_gap / mip_absolute_gap go into the header. Returns
+ * 0, or -1 with the reason in error (error_len bytes, may be NULL). */
+int sankhya_write_solution(const sankhya_model* model, const char* options, const char* path,
+                           const sankhya_result* result, const double* col_value,
+                           const double* row_dual, const double* col_dual,
+                           const double* row_activity, char* error, int32_t error_len);
 
 /* The host preconditioner on its own (src/la/scaling.cpp:111-159): Dr (m), Dc (n), the scaled
  * CSC values (nnz), scaled cost/bounds. min_cost is sense * c. */

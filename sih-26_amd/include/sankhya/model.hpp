@@ -42,6 +42,7 @@ enum class LimitReason : std::uint8_t { kNone, kInterrupt, kTime, kIterations, k
 
 [[nodiscard]] const char* to_string(SolveStatus status) noexcept;
 [[nodiscard]] const char* to_string(LimitReason reason) noexcept;
+[[nodiscard]] const char* to_string(BasisStatus status) noexcept;
 [[nodiscard]] SolveStatus status_for(LimitReason reason) noexcept;
 
 /// Does a solve ending in this state hand back a point? (reference model.hpp:581-597)

@@ -21,5 +21,8 @@ inline constexpr double kPdhgLoose = 1e-4;
 inline constexpr double kPdhgTight = 1e-8;
 /// Integrality, used by the quality report on integer columns (tolerances.hpp:25).
 inline constexpr double kIntegrality = 1e-6;
+/// MIP gap targets; a solution file records them in its header (tolerances.hpp:28,31).
+inline constexpr double kMipRelativeGap = 1e-4;
+inline constexpr double kMipAbsoluteGap = 1e-6;
 
 }  // namespace sankhya::tol

@@ -107,6 +107,10 @@ def load_library() -> C.CDLL:
                                           np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
                                           _f64p]
     lib.sankhya_batch_free.argtypes = [vp]
+    lib.sankhya_model_set_names.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p),
+                                            C.POINTER(C.c_char_p)]
+    lib.sankhya_write_solution.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(Result), vp, vp,
+                                           vp, vp, C.c_char_p, C.c_int32]
     lib.sankhya_pdhg_scaling.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
                                          _f64p, _f64p]
     lib.sankhya_pdhg_device_open.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
@@ -196,6 +200,21 @@ class Model:
         self._lib.sankhya_model_dims(self._h, C.byref(m), C.byref(n), C.byref(nnz))
         return m.value, n.value, nnz.value
 
+    def set_names(self, name=None, source_path=None, col_names=None, row_names=None) -> None:
+        """Model name, source path and row/column names, as a solution file writes them."""
+        m, n, _ = self.dims
+
+        def names(seq, count):
+            if seq is None:
+                return None
+            if len(seq) != count:
+                raise ValueError(f"expected {count} names, got {len(seq)}")
+            return (C.c_char_p * max(count, 1))(*[None if v is None else v.encode() for v in seq])
+
+        self._lib.sankhya_model_set_names(self._h, None if name is None else name.encode(),
+                                          None if source_path is None else source_path.encode(),
+                                          names(col_names, n), names(row_names, m))
+
     def arrays(self) -> Dict[str, np.ndarray]:
         m, n, nnz = self.dims
         d = dict(colptr=np.zeros(n + 1, np.int32), rowidx=np.zeros(max(nnz, 1), np.int32),
@@ -260,6 +279,30 @@ def solve(model: Model, **options) -> Solution:
                     x[:n], y[:m], d[:n], act[:m], res.algorithm.decode(), res.message.decode(),
                     res.solve_seconds, res.primal_infeasibility, res.dual_infeasibility,
                     res.complementarity_violation, res.relative_gap, stats)
+
+
+def write_solution(model: Model, solution: Solution, path, **options) -> None:
+    """Write the solution file (the reference's .sol format, `certificate none`) for a
+    solution of `model`. Raises OSError when the file cannot be written."""
+    lib = load_library()
+    m, n, _ = model.dims
+    res = Result()
+    res.status = STATUS_NAMES.index(solution.status)
+    res.objective, res.dual_bound = solution.objective, solution.dual_bound
+    res.iterations, res.solve_seconds = solution.iterations, solution.solve_seconds
+    res.primal_infeasibility = solution.primal_infeasibility
+    res.dual_infeasibility = solution.dual_infeasibility
+    res.algorithm = solution.algorithm.encode()[:31]
+    res.message = solution.message.encode()[:1023]
+    vec = lambda v, k: np.ascontiguousarray(v, dtype=np.float64) if v is not None and k else None
+    x, d = vec(solution.col_value, n), vec(solution.col_dual, n)
+    y, act = vec(solution.row_dual, m), vec(solution.row_activity, m)
+    ptr = lambda v: None if v is None else v.ctypes.data
+    err = C.create_string_buffer(1024)
+    rc = lib.sankhya_write_solution(model._h, _options_text(options), str(path).encode(),
+                                    C.byref(res), ptr(x), ptr(y), ptr(d), ptr(act), err, 1024)
+    if rc != 0:
+        raise OSError(err.value.decode())
 
 
 def _solution_of(res, x, y, d, act, stats_extra=None) -> Solution:

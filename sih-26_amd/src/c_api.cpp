@@ -11,6 +11,7 @@
 #include "../../include/sankhya_pdhg.h"
 #include "engine_stats.hpp"
 #include "generate.hpp"
+#include "sankhya/io.hpp"
 #include "sankhya/pdhg.hpp"
 #include "scaling.hpp"
 
@@ -288,6 +289,60 @@ int sankhya_pdhg_solve_batch(const sankhya_model* const* models, int32_t nlp, co
                                          row_activity, results);
   if (!s) return -1;
   sankhya_batch_free(s);
+  return 0;
+}
+
+int sankhya_model_set_names(sankhya_model* h, const char* name, const char* source_path,
+                            const char* const* col_names, const char* const* row_names) {
+  if (!h) return -1;
+  if (name) h->model.name = name;
+  if (source_path) h->model.source_path = source_path;
+  auto fill = [](std::vector<std::string>* out, const char* const* names, Index count) {
+    if (!names) return;
+    out->assign(static_cast<std::size_t>(count), std::string());
+    for (Index k = 0; k < count; ++k)
+      if (names[k]) (*out)[static_cast<std::size_t>(k)] = names[k];
+  };
+  fill(&h->model.col_names, col_names, h->model.num_cols());
+  fill(&h->model.row_names, row_names, h->model.num_rows());
+  return 0;
+}
+
+int sankhya_write_solution(const sankhya_model* h, const char* options, const char* path,
+                           const sankhya_result* result, const double* col_value,
+                           const double* row_dual, const double* col_dual,
+                           const double* row_activity, char* error, int32_t error_len) {
+  std::string err;
+  auto fail = [&](const std::string& e) {
+    if (error && error_len > 0) copy_text(error, static_cast<std::size_t>(error_len), e);
+    return -1;
+  };
+  if (!h || !path || !result) return fail("model, path and result are required");
+  Options o;
+  if (!apply_options(&o, options, &err)) return fail(err);
+  try {
+    const Index n = h->model.num_cols(), m = h->model.num_rows();
+    Solution s;
+    s.status = static_cast<SolveStatus>(result->status);
+    s.objective = result->objective;
+    s.dual_bound = result->dual_bound;
+    s.iterations = result->iterations;
+    s.solve_seconds = result->solve_seconds;
+    s.primal_infeasibility = result->primal_infeasibility;
+    s.dual_infeasibility = result->dual_infeasibility;
+    s.algorithm.assign(result->algorithm, strnlen(result->algorithm, sizeof(result->algorithm)));
+    s.message.assign(result->message, strnlen(result->message, sizeof(result->message)));
+    auto take = [](std::vector<double>* dst, const double* src, Index count) {
+      if (src) dst->assign(src, src + count);
+    };
+    take(&s.col_value, col_value, n);
+    take(&s.col_dual, col_dual, n);
+    take(&s.row_dual, row_dual, m);
+    take(&s.row_activity, row_activity, m);
+    if (!io::write_solution(path, h->model, s, o, &err)) return fail(err);
+  } catch (const std::exception& e) {
+    return fail(e.what());
+  }
   return 0;
 }
 

@@ -38,6 +38,18 @@ const char* to_string(LimitReason r) noexcept {
   return "none";
 }
 
+const char* to_string(BasisStatus b) noexcept {
+  switch (b) {
+    case BasisStatus::kUnknown: return "unknown";
+    case BasisStatus::kBasic: return "basic";
+    case BasisStatus::kAtLower: return "at_lower";
+    case BasisStatus::kAtUpper: return "at_upper";
+    case BasisStatus::kNonbasicFree: return "free";
+    case BasisStatus::kFixed: return "fixed";
+  }
+  return "unknown";
+}
+
 SolveStatus status_for(LimitReason r) noexcept {
   switch (r) {
     case LimitReason::kInterrupt: return SolveStatus::kInterrupted;

@@ -68,6 +68,11 @@ std::vector<OptionSpec> build_registry() {
                "fastest per LP before solving.", 0, 0});
   s.push_back({"pdhg_batch_tune_attempts", OptionType::Int, std::int64_t{96},
                "Batched engine: attempts per geometry when tuning the schedule.", 1.0, 100000.0});
+  s.push_back({"mip_relative_gap", OptionType::Double, tol::kMipRelativeGap,
+               "MIP relative gap target; this engine solves LPs and only records it in solution "
+               "files.", 0.0, 1.0});
+  s.push_back({"mip_absolute_gap", OptionType::Double, tol::kMipAbsoluteGap,
+               "MIP absolute gap target; recorded in solution files.", 0.0, kNoLimit});
   s.push_back({"pdhg_chunk", OptionType::Int, std::int64_t{0},
                "Attempts per captured graph chunk; 0 picks the default.", 0.0, 4096.0});
   s.push_back({"log_level", OptionType::String, std::string("info"), "off, error, warning, info, verbose, debug.",

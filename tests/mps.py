@@ -153,4 +153,5 @@ def read_mps(path: str) -> Dict:
     return dict(name=name, m=m, n=n, colptr=np.array(colptr, np.int32),
                 rowidx=np.array(rowidx, np.int32), val=np.array(val, np.float64), cost=c,
                 col_lower=cl, col_upper=cu, row_lower=rl, row_upper=ru, objective_offset=offset,
-                maximize=False, cost_nonzeros=int(np.count_nonzero(c)))
+                maximize=False, cost_nonzeros=int(np.count_nonzero(c)),
+                row_names=sorted(row_index, key=row_index.get), col_names=list(col_order))
