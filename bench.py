@@ -21,6 +21,9 @@ FP64" half:
     nonzeros per column, relative KKT <= 1e-6) solved in one batch in mixed precision, against
     plain FP64 single-LP solves of a 16-LP sample of the same LPs (their LPs/s extrapolated to
     the batch); `speedup` is the ratio of the two rates.
+And `adjoint`: 64 blend-style LPs (m = 5,000, n = 12,000) solved at relative KKT <= 1e-9, then
+the adjoint: forward and backward seconds, and the envelope error of L = c.x* (dL/db against
+y*) over the LPs with a square active system (see adjoint_check for degenerate vertices).
 
 Prints one JSON line. One GPU only: --gpus above 1 exits with a message.
 """
@@ -136,6 +139,74 @@ def batched_comparison(sh, count):
             "speedup": (solved / tb) / single_rate if single_rate > 0 else None}
 
 
+def adjoint_check(sh, count):
+    """The differentiable-LP workload: `count` blend LPs (m = 5,000, n = 12,000, seeds 5000+i)
+    solved in one batch at relative KKT <= 1e-9 (iteration limit 1,000,000), then the adjoint
+    (the backward of the torch layer) for L = c.x* and for L = w.x* (w ~ N(0,1)). Envelope check
+    of the first, dL/db against the row duals y*, relative to the largest entry, on every LP that
+    converged and whose active system is square (free columns as many as rows with entries: an
+    empty row, 0 = 0, leaves A one rank short); a degenerate vertex (fewer free columns than
+    rows: a basic column at its bound) has no unique dual, and there the check is that dL/db is
+    an exact dual of the active set, A_F^T (dL/db) = c_F."""
+    import numpy as np
+    import scipy.sparse as sps
+    import sankhya_hip.workloads as wl
+
+    t = time.perf_counter()
+    models = wl.blend(count)
+    datas = [m.arrays() for m in models]
+    log(f"[bench] adjoint workload: {count} blend LPs (generated in {time.perf_counter()-t:.1f}s)")
+    t = time.perf_counter()
+    ses = sh.BatchSession(models, pdhg_tolerance=1e-9, pdhg_stop_rule="relative_kkt",
+                          iteration_limit=1_000_000, pdhg_polish=False)
+    tf = time.perf_counter() - t
+    sols = ses.solutions
+    solved = [s.status in ("optimal", "feasible") for s in sols]
+    log(f"[bench] adjoint forward: {sum(solved)}/{count} solved in {tf:.2f}s")
+    t = time.perf_counter()
+    gc, gb, its, _ = ses.adjoint([d["cost"] for d in datas], None)
+    te = time.perf_counter() - t
+    rng = np.random.default_rng(7)
+    t = time.perf_counter()
+    ses.adjoint([rng.standard_normal(d["n"]) for d in datas], None)
+    tw = time.perf_counter() - t
+    env_b, env_c, deg_resid, square, degenerate, env_all = 0.0, 0.0, 0.0, 0, 0, []
+    for k, (d, s) in enumerate(zip(datas, sols)):
+        if not solved[k]:
+            continue
+        x, y = s.col_value, s.row_dual
+        lo, hi = d["col_lower"], d["col_upper"]
+        slack = np.minimum(np.where(np.abs(lo) < 1e30, x - lo, np.inf),
+                           np.where(np.abs(hi) < 1e30, hi - x, np.inf))
+        free = np.where(slack > np.abs(s.col_dual))[0]
+        env_c = max(env_c, float(np.abs(gc[k]).max()))  # dL/dc - x* for L = c.x*
+        if len(free) == int(np.count_nonzero(np.bincount(d["rowidx"], minlength=d["m"]))):
+            square += 1
+            env_all.append(float(np.abs(gb[k] - y).max() / max(1.0, np.abs(y).max())))
+            env_b = max(env_b, env_all[-1])
+        else:
+            degenerate += 1
+            # on the columns free beyond doubt (the tie column may read either way)
+            sure = np.where(slack > 10.0 * np.abs(s.col_dual))[0]
+            A = sps.csc_matrix((d["val"], d["rowidx"], d["colptr"]), shape=(d["m"], d["n"]))
+            cf = d["cost"][sure]
+            deg_resid = max(deg_resid, float(np.abs(A[:, sure].T @ gb[k] - cf).max() /
+                                             max(1.0, np.abs(cf).max())))
+    ses.close()
+    log(f"[bench] adjoint backward: c.x* {te:.2f}s, w.x* {tw:.2f}s; envelope error on {square} "
+        f"square LPs {env_b:.1e}; {degenerate} degenerate (active-system residual {deg_resid:.1e})")
+    return {"workload": f"blend-{count}: m = 5000, n = 12000, 40% of columns upper-bounded, "
+                        "relative KKT <= 1e-9, FP64 adjoint",
+            "lps": count, "solved": int(sum(solved)), "forward_seconds": tf,
+            "backward_seconds_objective": te, "backward_seconds_wx": tw,
+            "cgls_iterations_median": int(np.median(its[:, 0])),
+            "square_lps": square, "degenerate_lps": degenerate,
+            "envelope_rel_error_b": env_b,
+            "envelope_rel_error_b_median": float(np.median(env_all)) if env_all else None,
+            "envelope_abs_error_c": env_c,
+            "degenerate_active_system_residual": deg_resid, "envelope_target": 1e-5}
+
+
 def host_info():
     try:
         model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
@@ -158,6 +229,8 @@ def main():
     ap.add_argument("--batched-lps", type=int, default=512,
                     help="LPs in the batched comparison (0 skips it)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision solve")
+    ap.add_argument("--adjoint-lps", type=int, default=64,
+                    help="blend LPs for the adjoint measurement (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -230,6 +303,10 @@ def main():
     if args.batched_lps > 0:
         batched = batched_comparison(sh, args.batched_lps)
 
+    adjoint = None
+    if args.adjoint_lps > 0:
+        adjoint = adjoint_check(sh, args.adjoint_lps)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline (oracle, one core) on a bounded sample")
@@ -262,6 +339,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed": mixed,
         "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
 

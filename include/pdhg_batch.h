@@ -100,6 +100,10 @@ int pdhg_batch_adjoint(pdhg_batch* h, const double* grad_x, const double* grad_y
                        int max_iterations, double* grad_b, double* grad_c, int32_t* iterations,
                        double* residuals);
 
+/* Test hook: the active set the last pdhg_batch_adjoint call used, 1.0 / 0.0 per coordinate:
+ * mask_c (sum n) marks free columns, mask_r (sum m) active rows. */
+int pdhg_batch_adjoint_masks(pdhg_batch* h, double* mask_c, double* mask_r);
+
 #ifdef __cplusplus
 }
 #endif

@@ -96,6 +96,10 @@ int sankhya_batch_adjoint(const sankhya_batch* session, const double* grad_col_v
                           double* grad_cost, double* grad_row_bound, int32_t* iterations,
                           double* residuals);
 
+/* Test hook: the active set of the last sankhya_batch_adjoint call, in model order: mask_c
+ * (sum n) 1.0 on free columns, mask_r (sum m) 1.0 on active rows, 0.0 elsewhere. */
+int sankhya_batch_adjoint_masks(const sankhya_batch* session, double* mask_c, double* mask_r);
+
 void sankhya_batch_free(sankhya_batch* session);
 
 /* Write the solution file, the engine's certificate output (replaces io::write_solution,

@@ -1688,4 +1688,18 @@ int pdhg_batch_adjoint(pdhg_batch* h, const double* grad_x, const double* grad_y
   return PDHG_OK;
 }
 
+int pdhg_batch_adjoint_masks(pdhg_batch* h, double* mask_c, double* mask_r) {
+  int rc = check(h, true);
+  if (rc) return rc;
+  if (!mask_c || !mask_r) return bfail(h, PDHG_E_ARGUMENT, "null mask output");
+  if (!h->adj_mc) return bfail(h, PDHG_E_STATE, "no adjoint has run on this batch");
+  BHIP_TRY(h, hipSetDevice(h->device));
+  BHIP_TRY(h, hipMemcpyAsync(mask_c, h->adj_mc, sizeof(double) * static_cast<size_t>(h->sum_n),
+                             hipMemcpyDeviceToHost, h->stream));
+  BHIP_TRY(h, hipMemcpyAsync(mask_r, h->adj_mr, sizeof(double) * static_cast<size_t>(h->sum_m),
+                             hipMemcpyDeviceToHost, h->stream));
+  BHIP_TRY(h, hipStreamSynchronize(h->stream));
+  return PDHG_OK;
+}
+
 }  // extern "C"

@@ -107,6 +107,7 @@ def load_library() -> C.CDLL:
                                           np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
                                           _f64p]
     lib.sankhya_batch_free.argtypes = [vp]
+    lib.sankhya_batch_adjoint_masks.argtypes = [vp, _f64p, _f64p]
     lib.sankhya_model_set_names.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p),
                                             C.POINTER(C.c_char_p)]
     lib.sankhya_write_solution.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(Result), vp, vp,
@@ -364,6 +365,22 @@ class BatchSession:
             on += n
             om += m
         return out_c, out_b, its.reshape(-1, 2)[: len(self.dims)], res.reshape(-1, 2)[: len(self.dims)]
+
+    def adjoint_masks(self):
+        """The active set of the last adjoint call: (free-column masks, active-row masks), one
+        pair of 0/1 arrays per LP."""
+        sn, sm = sum(dm[1] for dm in self.dims), sum(dm[0] for dm in self.dims)
+        mc, mr = np.zeros(max(sn, 1)), np.zeros(max(sm, 1))
+        rc = self._lib.sankhya_batch_adjoint_masks(self._h, mc, mr)
+        if rc != 0:
+            raise RuntimeError(f"adjoint masks failed with code {rc}")
+        out_c, out_r, on, om = [], [], 0, 0
+        for m, n, _ in self.dims:
+            out_c.append(mc[on:on + n].copy())
+            out_r.append(mr[om:om + m].copy())
+            on += n
+            om += m
+        return out_c, out_r
 
     def close(self):
         if getattr(self, "_h", None):

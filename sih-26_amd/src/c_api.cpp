@@ -280,6 +280,11 @@ int sankhya_batch_adjoint(const sankhya_batch* session, const double* grad_col_v
                                    grad_row_bound, iterations, residuals);
 }
 
+int sankhya_batch_adjoint_masks(const sankhya_batch* session, double* mask_c, double* mask_r) {
+  if (!session || !session->session || !mask_c || !mask_r) return PDHG_E_ARGUMENT;
+  return session->session->adjoint_masks(mask_c, mask_r);
+}
+
 void sankhya_batch_free(sankhya_batch* session) { delete session; }
 
 int sankhya_pdhg_solve_batch(const sankhya_model* const* models, int32_t nlp, const char* options,

@@ -40,6 +40,8 @@ struct BatchSession {
   // model order (user-facing signs: the model's own sense and row duals).
   int adjoint(const double* grad_x, const double* grad_row_dual, double tol, int max_iterations,
               double* grad_c, double* grad_bound, int32_t* iterations, double* residuals) const;
+  // The active set of the last adjoint, in model order (1 free column / active row, else 0).
+  int adjoint_masks(double* mask_c, double* mask_r) const;
 };
 std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models,
                                          const Options& options, Logger& logger,

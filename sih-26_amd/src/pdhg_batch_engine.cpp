@@ -208,6 +208,36 @@ std::vector<Solution> run_batch(const std::vector<const Model*>& models, const O
   return open_batch(models, options, logger, control)->solutions;
 }
 
+int BatchSession::adjoint_masks(double* mask_c, double* mask_r) const {
+  std::vector<std::int64_t> off_n(models.size() + 1, 0), off_m(models.size() + 1, 0);
+  for (std::size_t k = 0; k < models.size(); ++k) {
+    off_n[k + 1] = off_n[k] + (models[k] ? models[k]->num_cols() : 0);
+    off_m[k + 1] = off_m[k] + (models[k] ? models[k]->num_rows() : 0);
+  }
+  std::fill(mask_c, mask_c + off_n.back(), 0.0);
+  std::fill(mask_r, mask_r + off_m.back(), 0.0);
+  if (!handle || live.empty()) return PDHG_E_STATE;
+  std::size_t sn = 0, sm = 0;
+  for (const std::size_t k : live) {
+    sn += static_cast<std::size_t>(off_n[k + 1] - off_n[k]);
+    sm += static_cast<std::size_t>(off_m[k + 1] - off_m[k]);
+  }
+  std::vector<double> mc(sn), mr(sm);
+  const int rc = pdhg_batch_adjoint_masks(handle, mc.data(), mr.data());
+  if (rc != PDHG_OK) return rc;
+  std::size_t pn = 0, pm = 0;
+  for (const std::size_t k : live) {
+    const std::int64_t n = off_n[k + 1] - off_n[k], m = off_m[k + 1] - off_m[k];
+    std::copy(mc.begin() + static_cast<std::ptrdiff_t>(pn), mc.begin() + static_cast<std::ptrdiff_t>(pn + n),
+              mask_c + off_n[k]);
+    std::copy(mr.begin() + static_cast<std::ptrdiff_t>(pm), mr.begin() + static_cast<std::ptrdiff_t>(pm + m),
+              mask_r + off_m[k]);
+    pn += static_cast<std::size_t>(n);
+    pm += static_cast<std::size_t>(m);
+  }
+  return PDHG_OK;
+}
+
 int BatchSession::adjoint(const double* grad_x, const double* grad_row_dual, double tol,
                           int max_iterations, double* grad_c, double* grad_bound,
                           int32_t* iterations, double* residuals) const {

@@ -5,7 +5,10 @@ Workload: blend-style LPs (m = 5,000 equality rows, one of them dense, n = 12,00
 nonzeros per column plus the dense row, 40% of the columns with upper bounds U[1,50],
 c ~ U[-5,5]; seeds 5000 + i). Checks:
   - the envelope identities through torch autograd: for L = c.x*, dL/dc = x* and
-    dL/db = the row duals, within 1e-5 relative of their largest entry;
+    dL/db = the row duals, within 1e-5 relative of their largest entry, on every LP whose
+    active system is square; on a degenerate vertex (a basic column at its bound, fewer free
+    columns than rows) the duals are not unique, and dL/db must instead be an exact dual of
+    the active set, A_F^T (dL/db) = c_F;
   - the device CGLS answers against direct sparse-LU solves on the solution's active set (scipy,
     CPU): dL/db for L = w.x* and dL/dc for L = v.row_dual, every entry within 1e-8 of the
     gradient's largest entry; and central finite differences (h = 1e-6) of those exact solution
@@ -46,6 +49,22 @@ def device():
 
 def blend(count, m=5000, n=12000, seed0=5000):
     return wl.blend(count, m, n, seed0)
+
+
+def _reduced_cost(d, row_dual):
+    """c - A^T row_dual: the engine's col_dual for a minimise model."""
+    A = sp.csc_matrix((d["val"], d["rowidx"], d["colptr"]), shape=(d["m"], d["n"]))
+    return d["cost"] - A.T @ row_dual
+
+
+def free_columns(d, x, reduced):
+    """The device's complementarity rule: free when the distance to the nearest bound exceeds
+    the reduced cost."""
+    lo, hi = d["col_lower"], d["col_upper"]
+    with np.errstate(invalid="ignore"):
+        slack = np.minimum(np.where(np.abs(lo) < 1e30, x - lo, np.inf),
+                           np.where(np.abs(hi) < 1e30, hi - x, np.inf))
+    return np.where(slack > np.abs(reduced))[0]
 
 
 def active_maps(d, sol):
@@ -99,12 +118,34 @@ def test_envelope_gradients_through_the_torch_layer():
     obj.backward()
     tb = time.time() - t
     print(f"forward (4 LPs) {tf:.2f}s, backward {tb:.3f}s", flush=True)
+    square = 0
     for k in range(4):
         xk, yk = x[k].detach().numpy(), y[k].detach().numpy()
         ex = np.abs(c.grad[k].numpy() - xk).max() / max(1.0, np.abs(xk).max())
         ey = np.abs(b.grad[k].numpy() - yk).max() / max(1.0, np.abs(yk).max())
-        print(f"  LP {k}: |dL/dc - x*| {ex:.1e}, |dL/db - y*| {ey:.1e}", flush=True)
-        assert ex <= 1e-5 and ey <= 1e-5
+        d = fam.templates[k]
+        F = free_columns(d, xk, _reduced_cost(d, yk))
+        print(f"  LP {k}: |F| {len(F)} for {d['m']} rows; |dL/dc - x*| {ex:.1e}, |dL/db - y*| {ey:.1e}",
+              flush=True)
+        assert ex <= 1e-5
+        # an empty row (possible in the generator's random pattern) is 0 = 0 and leaves A one
+        # rank short; the active system is square when F matches the rows that have entries
+        if len(F) == int(np.count_nonzero(np.bincount(d["rowidx"], minlength=d["m"]))):
+            square += 1
+            assert ey <= 1e-5
+        else:
+            # a degenerate vertex (a basic column at its bound): the row duals are not unique
+            # and dL/db is a subgradient; the adjoint's is still an exact dual of the active
+            # set, A_F^T (dL/db) = c_F
+            # (checked on the columns that are free beyond doubt, slack > 10 |reduced cost|: at
+            # a degenerate vertex the tie column can read either way in the last digits)
+            A = sp.csc_matrix((d["val"], d["rowidx"], d["colptr"]), shape=(d["m"], d["n"]))
+            Fs = free_columns(d, xk, 10.0 * _reduced_cost(d, yk))
+            c_f = d["cost"][Fs]
+            resid = np.abs(A[:, Fs].T @ b.grad[k].numpy() - c_f).max() / np.abs(c_f).max()
+            print(f"    degenerate: active-system residual {resid:.1e} on {len(Fs)} columns", flush=True)
+            assert resid <= 1e-8
+    assert square >= 3
 
 
 def test_adjoint_against_finite_differences_of_the_exact_maps():
