@@ -20,7 +20,8 @@ FP64" half:
   - `batched`: 512 LPs of the batched workload (m log-uniform in [200, 20000], n = 2.5 m, 6
     nonzeros per column, relative KKT <= 1e-6) solved in one batch in mixed precision, against
     plain FP64 single-LP solves of a 16-LP sample of the same LPs (their LPs/s extrapolated to
-    the batch); `speedup` is the ratio of the two rates.
+    the batch); `speedup` is the ratio of the two rates. The batch runs with the schedule tuner
+    on (the engine's default); `tuner` says how it placed the LPs.
 And `adjoint`: 64 blend-style LPs (m = 5,000, n = 12,000) solved at relative KKT <= 1e-9, then
 the adjoint: forward and backward seconds, and the envelope error of L = c.x* (dL/db against
 y*) over the LPs with a square active system (see adjoint_check for degenerate vertices).
@@ -117,7 +118,7 @@ def batched_comparison(sh, count):
     log(f"[bench] batched workload: {count} LPs, nnz {sum(m.dims[2] for m in models)} "
         f"(generated in {time.perf_counter()-t:.1f}s)")
     t = time.perf_counter()
-    sols = sh.solve_batch(models, pdhg_precision="mixed", pdhg_batch_tune=False, **opts)
+    sols = sh.solve_batch(models, pdhg_precision="mixed", **opts)
     tb = time.perf_counter() - t
     solved = sum(1 for s in sols if s.status in ("optimal", "feasible"))
     log(f"[bench] batched mixed: {solved}/{count} solved in {tb:.2f}s")
@@ -129,9 +130,14 @@ def batched_comparison(sh, count):
     single_rate = single_ok / ts if ts > 0 else 0.0
     log(f"[bench] fp64 single-LP sample: {single_ok}/{len(sample)} in {ts:.2f}s")
     its = sorted(s.iterations for s in sols)
+    persistent = [s.stats["schedule"] for s in sols if s.stats["schedule"] >= 0]
     return {"workload": f"batched-{count}: m log-uniform [200, 20000], n = 2.5 m, 6 nnz/col, "
                         "relative KKT <= 1e-6",
             "value": solved / tb, "unit": "LPs/s", "precision": "mixed", "seconds": tb,
+            "tuner": {"cooperative": count - len(persistent),
+                      "persistent_by_geometry": [persistent.count(g) for g in range(3)],
+                      "seconds_outside_the_loop": (sols[0].solve_seconds - sols[0].stats["loop_seconds"])
+                      if sols else None},
             "solved": solved, "lps": count, "max_iterations": its[-1],
             "median_iterations": its[len(its) // 2],
             "fp64_single_value": single_rate, "fp64_single_sample": len(sample),
@@ -141,7 +147,7 @@ def batched_comparison(sh, count):
 
 def adjoint_check(sh, count):
     """The differentiable-LP workload: `count` blend LPs (m = 5,000, n = 12,000, seeds 5000+i)
-    solved in one batch at relative KKT <= 1e-9 (iteration limit 1,000,000), then the adjoint
+    solved in one batch at relative KKT <= 1e-9 (iteration limit 2,000,000), then the adjoint
     (the backward of the torch layer) for L = c.x* and for L = w.x* (w ~ N(0,1)). Envelope check
     of the first, dL/db against the row duals y*, relative to the largest entry, on every LP that
     converged and whose active system is square (free columns as many as rows with entries: an
@@ -158,7 +164,7 @@ def adjoint_check(sh, count):
     log(f"[bench] adjoint workload: {count} blend LPs (generated in {time.perf_counter()-t:.1f}s)")
     t = time.perf_counter()
     ses = sh.BatchSession(models, pdhg_tolerance=1e-9, pdhg_stop_rule="relative_kkt",
-                          iteration_limit=1_000_000, pdhg_polish=False)
+                          iteration_limit=2_000_000, pdhg_polish=False)
     tf = time.perf_counter() - t
     sols = ses.solutions
     solved = [s.status in ("optimal", "feasible") for s in sols]

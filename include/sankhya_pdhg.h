@@ -53,6 +53,8 @@ typedef struct {
   int32_t status; /* sankhya::SolveStatus */
   int64_t iterations, attempts, restarts, ticks, device_bytes;
   int64_t fp32_iterations; /* mixed precision: iterations run before the FP64 continuation */
+  int32_t schedule;        /* batched engine: -1 cooperative, else the persistent geometry the
+                              tuner chose (0: 256 threads, 1: 512, 2: 1024); -1 for one LP */
   double objective, dual_bound, solve_seconds, setup_seconds, loop_seconds, spectral_norm;
   double primal_infeasibility, dual_infeasibility, complementarity_violation, relative_gap;
   double final_primal, final_dual, final_gap;

@@ -35,6 +35,7 @@ class Result(C.Structure):
         ("status", C.c_int32),
         ("iterations", C.c_int64), ("attempts", C.c_int64), ("restarts", C.c_int64),
         ("ticks", C.c_int64), ("device_bytes", C.c_int64), ("fp32_iterations", C.c_int64),
+        ("schedule", C.c_int32),
         ("objective", C.c_double), ("dual_bound", C.c_double), ("solve_seconds", C.c_double),
         ("setup_seconds", C.c_double), ("loop_seconds", C.c_double),
         ("spectral_norm", C.c_double),
@@ -309,7 +310,7 @@ def write_solution(model: Model, solution: Solution, path, **options) -> None:
 def _solution_of(res, x, y, d, act, stats_extra=None) -> Solution:
     stats = {k: getattr(res, k) for k in ("attempts", "restarts", "ticks", "device_bytes", "fp32_iterations",
                                           "setup_seconds", "loop_seconds", "spectral_norm",
-                                          "final_primal", "final_dual", "final_gap")}
+                                          "final_primal", "final_dual", "final_gap", "schedule")}
     return Solution(STATUS_NAMES[res.status], res.objective, res.dual_bound, res.iterations,
                     x, y, d, act, res.algorithm.decode(), res.message.decode(),
                     res.solve_seconds, res.primal_infeasibility, res.dual_infeasibility,

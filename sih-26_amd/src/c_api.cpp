@@ -169,6 +169,7 @@ int sankhya_pdhg_solve(const sankhya_model* h, const char* options, double* col_
     out->ticks = st.ticks;
     out->device_bytes = st.device_bytes;
     out->fp32_iterations = st.fp32_iterations;
+    out->schedule = -1;
     out->objective = s.objective;
     out->dual_bound = s.dual_bound;
     out->solve_seconds = s.solve_seconds;
@@ -255,6 +256,7 @@ sankhya_batch* sankhya_batch_solve(const sankhya_model* const* models, int32_t n
       out->final_primal = st.final_primal;
       out->final_dual = st.final_dual;
       out->final_gap = st.final_gap;
+      out->schedule = st.schedule;
     }
     out->objective = s.objective;
     out->dual_bound = s.dual_bound;

@@ -16,6 +16,7 @@ namespace sankhya::hip_engine {
 struct EngineStats {
   std::int64_t iterations = 0, attempts = 0, restarts = 0, ticks = 0, device_bytes = 0;
   std::int64_t fp32_iterations = 0;  // mixed precision: iterations before the FP64 handoff
+  std::int32_t schedule = -1;  // batched engine: -1 cooperative, else the persistent geometry
   double spectral_norm = 0.0;
   double setup_seconds = 0.0, loop_seconds = 0.0, total_seconds = 0.0;
   double final_primal = 0.0, final_dual = 0.0, final_gap = 0.0;

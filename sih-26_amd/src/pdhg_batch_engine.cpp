@@ -124,8 +124,8 @@ std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models
   params.stop_rule = options.get_string("pdhg_stop_rule") == "relative_kkt" ? 1 : 0;
   logger.info("Solving {} LPs with MI355X batched PDHG on {}, precision {}", nb, desc,
               mixed ? "mixed" : "fp64");
+  std::vector<int32_t> geo(nb, -1);
   if (options.get_bool("pdhg_batch_tune")) {
-    std::vector<int32_t> geo(nb, 0);
     rc = pdhg_batch_tune(dev_h, &params, norms.data(),
                          static_cast<int>(options.get_int("pdhg_batch_tune_attempts")), geo.data());
     if (rc != PDHG_OK) return dev_error(rc, "schedule tuning");
@@ -191,6 +191,7 @@ std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models
     es.final_primal = res[9 * q + 0];
     es.final_dual = res[9 * q + 1];
     es.final_gap = res[9 * q + 2];
+    es.schedule = geo[q];
     report(*models[k], res.data() + 9 * q, yl, rl2,
            ReportState{st.done != 0, st.done ? SolveStatus::kOptimal : stop_status, st.iteration,
                        st.restarts, tolerance, stop_at_request},
