@@ -97,3 +97,32 @@ def c3_sizes(count: int, seed: int = 1000):
 def c3_batch(count: int, seed: int = 1000):
     """`count` LPs of the batched workload (sankhya_hip.workloads.batched)."""
     return wl.batched(count, seed)
+
+
+def simplex_result(d):
+    """The LP solved to a vertex by HiGHS's dual simplex (scipy.optimize.linprog): an FP64
+    simplex oracle independent of PDHG. Rows become equalities or one-sided inequalities."""
+    import scipy.optimize as so
+    import scipy.sparse as sps
+    A = sps.csc_matrix((d["val"], d["rowidx"], d["colptr"]), shape=(d["m"], d["n"]))
+    rl, ru = np.asarray(d["row_lower"], float), np.asarray(d["row_upper"], float)
+    fl, fu = np.abs(rl) < 1e30, np.abs(ru) < 1e30
+    eq = fl & fu & (rl == ru)
+    ub_hi = fu & ~eq
+    ub_lo = fl & ~eq
+    A_ub = sps.vstack([A[ub_hi], -A[ub_lo]]).tocsc() if (ub_hi.any() or ub_lo.any()) else None
+    b_ub = np.concatenate([ru[ub_hi], -rl[ub_lo]]) if A_ub is not None else None
+    cl, cu = np.asarray(d["col_lower"], float), np.asarray(d["col_upper"], float)
+    bounds = np.column_stack([np.where(np.abs(cl) < 1e30, cl, -np.inf), np.where(np.abs(cu) < 1e30, cu, np.inf)])
+    sign = -1.0 if d.get("maximize") else 1.0
+    r = so.linprog(sign * np.asarray(d["cost"], float), A_ub=A_ub, b_ub=b_ub,
+                   A_eq=A[eq] if eq.any() else None, b_eq=rl[eq] if eq.any() else None,
+                   bounds=bounds, method="highs-ds")
+    assert r.status == 0, r.message
+    return r, eq
+
+
+def simplex_objective(d) -> float:
+    r, _ = simplex_result(d)
+    sign = -1.0 if d.get("maximize") else 1.0
+    return sign * r.fun + float(d.get("objective_offset", 0.0))

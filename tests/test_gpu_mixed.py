@@ -5,7 +5,8 @@ matrix), and the FP64 continuation ("polish") that finishes the same run.
 The reference is FP64 only, so parity here is against the engine's own FP64 path and the
 oracle on the same inputs:
   - C0 (1,000 x 3,000) mixed with the polish to 1e-8 (stop at request): objective within 1e-7
-    relative of the FP64 solve and of the oracle, and the point accepted by tests/verify.py;
+    relative of the FP64 solve, of the oracle and of the FP64 simplex optimum (HiGHS, as
+    shipped in scipy), and the point accepted by tests/verify.py;
   - the 1M x 2M Zipf LP (nnz 20M) mixed to 1e-6 relative KKT: same status as the FP64 solve,
     objective within 1e-5 relative (both are 1e-6-accurate points; the difference printed is
     the error the mixed path reports against FP64);
@@ -50,6 +51,12 @@ def test_c0_mixed_polished_to_1e8_matches_fp64_and_oracle():
     assert abs(b.objective - r["objective"]) <= 1e-7 * max(1.0, abs(r["objective"]))
     ok, checks = accepted(d, _sol(b))
     assert ok, checks
+    # against the FP64 simplex optimum (HiGHS through scipy: an exact vertex)
+    exact = cm.simplex_objective(d)
+    err_mixed = abs(b.objective - exact) / max(1.0, abs(exact))
+    err_fp64 = abs(a.objective - exact) / max(1.0, abs(exact))
+    print(f"simplex optimum {exact:.15g}: mixed error {err_mixed:.1e}, fp64 error {err_fp64:.1e}", flush=True)
+    assert err_mixed <= 1e-7 and err_fp64 <= 1e-7
 
 
 def test_zipf_1m_2m_mixed_against_fp64():
