@@ -1172,14 +1172,56 @@ int set_bins(pdhg_batch* h) {
 // The cooperative grids over the cooperative LPs that are still running (`running` from the
 // host copy of the control blocks; all of them when `all` is set).
 int compact(pdhg_batch* h, bool all) {
-  std::vector<int32_t> lps, ia, it;
-  for (int32_t k = 0; k < h->nlp; ++k) {
-    if (!h->coop[k] || (!all && h->ctl_host[k].halt)) continue;
-    lps.push_back(k);
+  // XCD placement: grid position p runs on XCD p % kXcds, and each XCD has its own L2. Every
+  // running LP is given one XCD (largest LP first, to the least loaded), and its items take
+  // that XCD's positions, so an LP's iterate vectors are fetched into one L2 instead of all
+  // eight. The order of the items does not change any result: partial sums are indexed by
+  // item and added per LP in item order.
+  constexpr int kXcds = 8;
+  std::vector<int32_t> lps;
+  for (int32_t k = 0; k < h->nlp; ++k)
+    if (h->coop[k] && (all || !h->ctl_host[k].halt)) lps.push_back(k);
+  auto work = [&](int32_t k) {
     const LpDesc& d = h->desc_host[k];
-    for (int32_t q = 0; q < d.a[0].nitems; ++q) ia.push_back(d.a[0].item0 + q);
-    for (int32_t q = 0; q < d.t[0].nitems; ++q) it.push_back(d.t[0].item0 + q);
+    return int64_t{d.a[0].nitems} + d.t[0].nitems;
+  };
+  std::vector<int32_t> by_size(lps);
+  std::stable_sort(by_size.begin(), by_size.end(),
+                   [&](int32_t a, int32_t b) { return work(a) > work(b); });
+  std::vector<std::vector<int32_t>> lps_of(kXcds);
+  std::array<int64_t, kXcds> load{};
+  for (const int32_t k : by_size) {
+    int x = 0;
+    for (int c = 1; c < kXcds; ++c)
+      if (load[c] < load[x]) x = c;
+    lps_of[x].push_back(k);
+    load[x] += work(k);
   }
+  // Items of one product: each XCD's queue holds its LPs' items LP after LP; position p takes
+  // the next item of queue p % kXcds, or of the fullest queue once that one is empty.
+  auto place = [&](bool rows_of_a) {
+    std::vector<std::vector<int32_t>> q(kXcds);
+    size_t total = 0;
+    for (int x = 0; x < kXcds; ++x)
+      for (const int32_t k : lps_of[x]) {
+        const LpPlan& pd = rows_of_a ? h->desc_host[k].a[0] : h->desc_host[k].t[0];
+        for (int32_t i = 0; i < pd.nitems; ++i) q[x].push_back(pd.item0 + i);
+        total += static_cast<size_t>(pd.nitems);
+      }
+    std::vector<int32_t> out;
+    out.reserve(total);
+    std::array<size_t, kXcds> next{};
+    for (size_t p = 0; p < total; ++p) {
+      int x = static_cast<int>(p % kXcds);
+      if (next[x] >= q[x].size()) {
+        for (int c = 0; c < kXcds; ++c)
+          if (q[c].size() - next[c] > q[x].size() - next[x]) x = c;
+      }
+      out.push_back(q[x][next[x]++]);
+    }
+    return out;
+  };
+  const std::vector<int32_t> ia = place(true), it = place(false);
   h->n_act_lp = static_cast<int32_t>(lps.size());
   h->n_act_a = static_cast<int32_t>(ia.size());
   h->n_act_t = static_cast<int32_t>(it.size());

@@ -18,6 +18,9 @@ namespace sankhya {
 
 enum class LogLevel { kQuiet = 0, kError = 1, kWarning = 2, kInfo = 3, kVerbose = 4, kDebug = 5 };
 
+/// The `log_level` option's text (off, error, warning, info, verbose, debug) as a level.
+[[nodiscard]] LogLevel log_level_of(const std::string& text);
+
 namespace detail {
 
 inline std::string render_one(const std::string& spec, double v) {

@@ -210,7 +210,7 @@ sankhya_batch* sankhya_batch_solve(const sankhya_model* const* models, int32_t n
   }
   std::vector<const Model*> list(static_cast<std::size_t>(nlp));
   for (int32_t k = 0; k < nlp; ++k) list[k] = models[k] ? &models[k]->model : nullptr;
-  Logger logger(o.get_bool("log_to_console") ? stdout : nullptr, LogLevel::kInfo);
+  Logger logger(o.get_bool("log_to_console") ? stdout : nullptr, log_level_of(o.get_string("log_level")));
   auto* out_session = new sankhya_batch();
   try {
     out_session->session = hip_engine::open_batch(list, o, logger, nullptr);

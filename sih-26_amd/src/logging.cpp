@@ -7,6 +7,15 @@
 
 namespace sankhya {
 
+LogLevel log_level_of(const std::string& s) {
+  if (s == "off") return LogLevel::kQuiet;
+  if (s == "error") return LogLevel::kError;
+  if (s == "warning") return LogLevel::kWarning;
+  if (s == "verbose") return LogLevel::kVerbose;
+  if (s == "debug") return LogLevel::kDebug;
+  return LogLevel::kInfo;
+}
+
 Logger::~Logger() {
   if (progress_stream_) std::fclose(progress_stream_);
 }

@@ -150,8 +150,12 @@ std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models
       stop_status = SolveStatus::kInterrupted;
       break;
     }
+    const int32_t before = running;
     rc = pdhg_batch_run(dev_h, 1, &running);
     if (rc != PDHG_OK) return dev_error(rc, "iteration");
+    if (running != before)
+      logger.verbose("batch: {} of {} LPs running at {:.3f}s", running, nb,
+                     timer.elapsed_seconds() - loop_start);
   }
   const double loop_seconds = timer.elapsed_seconds() - loop_start;
 

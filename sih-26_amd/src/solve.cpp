@@ -10,20 +10,10 @@
 
 namespace sankhya {
 
-namespace {
-LogLevel level_of(const std::string& s) {
-  if (s == "off") return LogLevel::kQuiet;
-  if (s == "error") return LogLevel::kError;
-  if (s == "warning") return LogLevel::kWarning;
-  if (s == "verbose") return LogLevel::kVerbose;
-  if (s == "debug") return LogLevel::kDebug;
-  return LogLevel::kInfo;
-}
-}  // namespace
 
 Solution solve(const Model& model, const Options& options, SolveControl* control) {
   Logger logger(options.get_bool("log_to_console") ? stdout : nullptr,
-                level_of(options.get_string("log_level")));
+                log_level_of(options.get_string("log_level")));
   if (!options.get_string("progress_out").empty())
     logger.enable_progress_output(options.get_string("progress_out"));
   Solution solution;
