@@ -406,8 +406,7 @@ __global__ __launch_bounds__(kCT) void k_coop_tail(CoopArgs<V, X> C) {
   const bool has_lr_p = d.t[0].nlr > 0;
   auto partials = [&]() {
     double p[1] = {0.0}, q[2] = {0.0, 0.0};
-    lane_sums<1>(pp, np, 1, p);
-    lane_sums<2>(pd, nd, 2, q);
+    lane_sums_1_2(pp, np, pd, nd, p, q);
     if (has_lr_p && threadIdx.x == 0) p[0] += C.lr_p[lp];
     a.s[0] = p[0];
     a.s[1] = q[0];

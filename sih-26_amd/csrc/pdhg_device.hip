@@ -119,8 +119,7 @@ __global__ __launch_bounds__(kTail) void k_dual_tail(PlanView<V> A, Iter<X> it, 
   a.zero();
   auto partials = [&]() {
     double p[1] = {0.0}, d[2] = {0.0, 0.0};
-    lane_sums<1>(part_p, np, 1, p);
-    lane_sums<2>(part_d, nd, 2, d);
+    lane_sums_1_2(part_p, np, part_d, nd, p, d);
     if (has_lr_p && threadIdx.x == 0) p[0] += lr_p[0];
     a.s[0] = p[0];
     a.s[1] = d[0];

@@ -493,4 +493,31 @@ __device__ __forceinline__ void lane_sums(const double* p, int64_t count, int st
   }
 }
 
+// lane_sums over two arrays at once, one value per entry of a (count ca) and two per entry of b
+// (count cb): every load of a round is issued before any is used, so both arrays share each
+// memory round trip. Every lane keeps its own partial sums.
+__device__ __forceinline__ void lane_sums_1_2(const double* a, int64_t ca, const double* b,
+                                              int64_t cb, double (&sa)[1], double (&sb)[2]) {
+  constexpr int U = 4;
+  const int64_t most = ca > cb ? ca : cb;
+  const int64_t rounds = (most + static_cast<int64_t>(blockDim.x) * U - 1) /
+                         (static_cast<int64_t>(blockDim.x) * U);
+  for (int64_t r = 0; r < rounds; ++r) {
+    double va[U], vb[U][2];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t q = (r * U + u) * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+      va[u] = q < ca ? a[q] : 0.0;
+      vb[u][0] = q < cb ? b[2 * q] : 0.0;
+      vb[u][1] = q < cb ? b[2 * q + 1] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      sa[0] += va[u];
+      sb[0] += vb[u][0];
+      sb[1] += vb[u][1];
+    }
+  }
+}
+
 }  // namespace sankhya_hip
