@@ -115,8 +115,9 @@ int sankhya_write_solution(const sankhya_model* model, const char* options, cons
                            const double* row_activity, char* error, int32_t error_len);
 
 /* The host preconditioner on its own (src/la/scaling.cpp:111-159): Dr (m), Dc (n), the scaled
- * CSC values (nnz), scaled cost/bounds. min_cost is sense * c. */
-int sankhya_pdhg_scaling(const sankhya_model* model, int ruiz_iterations, double* row,
+ * CSC values (nnz), scaled cost/bounds. min_cost is sense * c. threads: host threads for the
+ * passes (same result bit for bit; models under 200,000 nonzeros use one). */
+int sankhya_pdhg_scaling(const sankhya_model* model, int ruiz_iterations, int threads, double* row,
                          double* column, double* scaled_val, double* cost_hat,
                          double* col_lower_hat, double* col_upper_hat, double* row_lower_hat,
                          double* row_upper_hat);

@@ -113,8 +113,8 @@ def load_library() -> C.CDLL:
                                             C.POINTER(C.c_char_p)]
     lib.sankhya_write_solution.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(Result), vp, vp,
                                            vp, vp, C.c_char_p, C.c_int32]
-    lib.sankhya_pdhg_scaling.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
-                                         _f64p, _f64p]
+    lib.sankhya_pdhg_scaling.argtypes = [vp, C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                         _f64p, _f64p, _f64p]
     lib.sankhya_pdhg_device_open.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
     lib.pdhg_dev_probe.argtypes = [C.c_int, C.c_char_p, C.c_int]
     lib.pdhg_dev_destroy.argtypes = [vp]
@@ -418,14 +418,15 @@ def solve_batch(models, **options):
     return out
 
 
-def host_scaling(model: Model, ruiz: int = 10) -> Dict[str, np.ndarray]:
-    """The engine's host preconditioner (Dr, Dc, scaled CSC values, scaled vectors)."""
+def host_scaling(model: Model, ruiz: int = 10, threads: int = 1) -> Dict[str, np.ndarray]:
+    """The engine's host preconditioner (Dr, Dc, scaled CSC values, scaled vectors), on
+    `threads` host threads."""
     lib = load_library()
     m, n, nnz = model.dims
     out = dict(row=np.zeros(max(m, 1)), column=np.zeros(max(n, 1)), val=np.zeros(max(nnz, 1)),
                cost=np.zeros(max(n, 1)), col_lower=np.zeros(max(n, 1)), col_upper=np.zeros(max(n, 1)),
                row_lower=np.zeros(max(m, 1)), row_upper=np.zeros(max(m, 1)))
-    lib.sankhya_pdhg_scaling(model._h, ruiz, out["row"], out["column"], out["val"], out["cost"],
+    lib.sankhya_pdhg_scaling(model._h, ruiz, int(threads), out["row"], out["column"], out["val"], out["cost"],
                              out["col_lower"], out["col_upper"], out["row_lower"], out["row_upper"])
     for k, ln in (("row", m), ("column", n), ("val", nnz), ("cost", n), ("col_lower", n),
                   ("col_upper", n), ("row_lower", m), ("row_upper", m)):

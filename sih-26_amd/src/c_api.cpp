@@ -9,6 +9,7 @@
 #include <string>
 
 #include "../../include/sankhya_pdhg.h"
+#include "engine_common.hpp"
 #include "engine_stats.hpp"
 #include "generate.hpp"
 #include "sankhya/io.hpp"
@@ -353,13 +354,13 @@ int sankhya_write_solution(const sankhya_model* h, const char* options, const ch
   return 0;
 }
 
-int sankhya_pdhg_scaling(const sankhya_model* h, int ruiz, double* row, double* column,
+int sankhya_pdhg_scaling(const sankhya_model* h, int ruiz, int threads, double* row, double* column,
                          double* scaled_val, double* cost_hat, double* col_lower_hat,
                          double* col_upper_hat, double* row_lower_hat, double* row_upper_hat) {
   const Model& md = h->model;
   std::vector<double> cost(md.col_cost.size());
   for (std::size_t j = 0; j < cost.size(); ++j) cost[j] = md.sense_multiplier() * md.col_cost[j];
-  const hip_engine::Scaling s = hip_engine::build_scaling(md, cost, ruiz);
+  const hip_engine::Scaling s = hip_engine::build_scaling(md, cost, ruiz, threads);
   auto cp = [](double* dst, const std::vector<double>& src) {
     if (dst && !src.empty()) std::memcpy(dst, src.data(), sizeof(double) * src.size());
   };
@@ -387,7 +388,7 @@ int sankhya_pdhg_device_open(const sankhya_model* h, int device, int precision, 
                      : (is_finite_bound(md.row_upper[i]) ? md.row_upper[i] : 0.0);
     bs += b * b;
   }
-  const hip_engine::Scaling s = hip_engine::build_scaling(md, cost, 10);
+  const hip_engine::Scaling s = hip_engine::build_scaling(md, cost, 10, hip_engine::host_threads());
   // CSR of Ahat by counting transpose.
   const auto& cp = md.matrix.column_starts();
   const auto& ri = md.matrix.row_indices();

@@ -48,7 +48,7 @@ void csc_to_csr(Index m, Index n, const std::vector<Index>& cp, const std::vecto
   }
 }
 
-Prepared prepare(const Model& model) {
+Prepared prepare(const Model& model, int threads) {
   Prepared p;
   const Index rows = model.num_rows(), cols = model.num_cols();
   const double sense = model.sense_multiplier();
@@ -68,7 +68,7 @@ Prepared prepare(const Model& model) {
   }
   p.bound_norm = std::sqrt(bound_square);
   // Preconditioning (pdhg.cpp:113-117) and the CSR of Ahat the device reads next to the CSC.
-  p.scaling = build_scaling(model, p.cost, kRuizIterations);
+  p.scaling = build_scaling(model, p.cost, kRuizIterations, threads);
   csc_to_csr(rows, cols, model.matrix.column_starts(), model.matrix.row_indices(),
              p.scaling.values, &p.a_rp, &p.a_ci, &p.a_v);
   return p;
@@ -142,6 +142,11 @@ int device_from_option(const std::string& text, Logger& logger) {
 }
 
 }  // namespace
+
+int host_threads() {
+  const unsigned hw = std::thread::hardware_concurrency();
+  return static_cast<int>(std::min(16u, hw == 0 ? 1u : hw));
+}
 
 int engine_device(const Options& options, Logger& logger) {
   return device_from_option(options.get_string("gpu_devices"), logger);

@@ -40,7 +40,8 @@ struct Prepared {
   std::vector<Index> a_rp, a_ci;  // CSR of Ahat
   std::vector<double> a_v;
 };
-Prepared prepare(const Model& model);
+// threads: host threads for the preconditioner (the batch prepares LPs in parallel with 1).
+Prepared prepare(const Model& model, int threads = 1);
 
 std::vector<double> power_start(const Model& model, std::int64_t random_seed);
 
@@ -58,6 +59,9 @@ void report(const Model& model, const double* res9, const std::vector<double>& y
 
 // The device the gpu_devices option names (one GPU: the first listed, with a warning when
 // several are given).
+// Host threads for one LP's preparation: the machine's, at most 16.
+int host_threads();
+
 int engine_device(const Options& options, Logger& logger);
 
 // prepare() and power_start() for every model, on all host cores (one model per task).

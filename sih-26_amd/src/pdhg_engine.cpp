@@ -72,7 +72,7 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   const double sense = model.sense_multiplier();
   const auto n = static_cast<std::size_t>(cols), m = static_cast<std::size_t>(rows);
 
-  const Prepared prep = prepare(model);
+  const Prepared prep = prepare(model, host_threads());
   const Scaling& scaling = prep.scaling;
   const std::vector<double>& cost = prep.cost;
   g_last_stats.setup_seconds = timer.elapsed_seconds();

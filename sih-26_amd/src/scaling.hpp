@@ -19,7 +19,9 @@ struct Scaling {
   double max_abs = 0.0, min_abs = 0.0;
 };
 
+// threads > 1 splits the passes over that many host threads with the same result bit for
+// bit (models under 200,000 nonzeros always use one).
 [[nodiscard]] Scaling build_scaling(const Model& model, const std::vector<double>& min_space_cost,
-                                    int ruiz_iterations);
+                                    int ruiz_iterations, int threads = 1);
 
 }  // namespace sankhya::hip_engine

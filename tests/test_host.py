@@ -67,6 +67,20 @@ def test_host_scaling_matches_oracle_bit_for_bit(name, inst):
         assert np.array_equal(a[k], b[k]), k
 
 
+@pytest.mark.parametrize("kind,m,n,per_col", [("zipf", 20000, 50000, 10), ("uniform", 40000, 100000, 5),
+                                               ("blend", 30000, 72000, 5)])
+def test_threaded_host_scaling_is_bit_identical(kind, m, n, per_col):
+    """The preconditioner split over host threads (row maxima merged, Pock-Chambolle row sums
+    taken row by row in column order) gives the single-threaded answer bit for bit."""
+    mdl = sh.Model.generate(kind, m, n, per_col, 9)
+    assert mdl.dims[2] >= 200000  # large enough that the threaded path runs
+    a = sh.host_scaling(mdl, threads=1)
+    for t in (3, 8):
+        b = sh.host_scaling(mdl, threads=t)
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (t, k)
+
+
 def test_generators_are_deterministic_and_shaped():
     m1, d1 = cm.c0(4)
     m2, d2 = cm.c0(4)
