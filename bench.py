@@ -149,7 +149,8 @@ def adjoint_check(sh, count):
     """The differentiable-LP workload: `count` blend LPs (m = 5,000, n = 12,000, seeds 5000+i)
     solved in one batch at relative KKT <= 1e-9 (iteration limit 2,000,000), then the adjoint
     (the backward of the torch layer) for L = c.x* and for L = w.x* (w ~ N(0,1)). Envelope check
-    of the first, dL/db against the row duals y*, relative to the largest entry, on every LP that
+    of the first, dL/db against the row duals y*, relative to the largest entry (and, beside it,
+    in the 2-norm), on every LP that
     converged and whose active system is square (free columns as many as rows with entries: an
     empty row, 0 = 0, leaves A one rank short); a degenerate vertex (fewer free columns than
     rows: a basic column at its bound) has no unique dual, and there the check is that dL/db is
@@ -176,7 +177,7 @@ def adjoint_check(sh, count):
     t = time.perf_counter()
     ses.adjoint([rng.standard_normal(d["n"]) for d in datas], None)
     tw = time.perf_counter() - t
-    env_b, env_c, deg_resid, square, degenerate, env_all = 0.0, 0.0, 0.0, 0, 0, []
+    env_b, env_c, deg_resid, square, degenerate, env_all, env_l2 = 0.0, 0.0, 0.0, 0, 0, [], 0.0
     for k, (d, s) in enumerate(zip(datas, sols)):
         if not solved[k]:
             continue
@@ -189,6 +190,7 @@ def adjoint_check(sh, count):
         if len(free) == int(np.count_nonzero(np.bincount(d["rowidx"], minlength=d["m"]))):
             square += 1
             env_all.append(float(np.abs(gb[k] - y).max() / max(1.0, np.abs(y).max())))
+            env_l2 = max(env_l2, float(np.linalg.norm(gb[k] - y) / max(1.0, np.linalg.norm(y))))
             env_b = max(env_b, env_all[-1])
         else:
             degenerate += 1
@@ -209,6 +211,7 @@ def adjoint_check(sh, count):
             "square_lps": square, "degenerate_lps": degenerate,
             "envelope_rel_error_b": env_b,
             "envelope_rel_error_b_median": float(np.median(env_all)) if env_all else None,
+            "envelope_rel_error_b_l2": env_l2,
             "envelope_abs_error_c": env_c,
             "degenerate_active_system_residual": deg_resid, "envelope_target": 1e-5}
 
