@@ -287,6 +287,16 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
         ein[q] = epi.load(r0 + rr);
       }
     }
+    // All gathers go out before any product is formed: a lane whose nonzero lies outside the
+    // item gathers column 0 (present in every matrix that has an item) and never uses it, so
+    // no branch surrounds a load and the gathers of all U steps are in flight together.
+    RowSum<NR> g0[U], g1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = kb + 2 * (u * kThreads + tid);
+      g0[u] = gather((k < k1 && k >= k0) ? cidx[u].x : 0);
+      g1[u] = gather((k + 1 < k1) ? cidx[u].y : 0);
+    }
     // Products of right-hand side 0 go to LDS; those of the others wait in registers and
     // reuse the same kTile doubles of LDS one after another (NR passes, one buffer).
     double hold[NR > 1 ? NR - 1 : 1][U][2];
@@ -295,16 +305,14 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
       const int k = kb + 2 * (u * kThreads + tid);
       if (k < k1) {
         if (k >= k0) {
-          const RowSum<NR> g = gather(cidx[u].x);
-          lds[k - k0] = aval[u].x * g.v[0];
+          lds[k - k0] = aval[u].x * g0[u].v[0];
 #pragma unroll
-          for (int r = 1; r < NR; ++r) hold[r - 1][u][0] = aval[u].x * g.v[r];
+          for (int r = 1; r < NR; ++r) hold[r - 1][u][0] = aval[u].x * g0[u].v[r];
         }
         if (k + 1 < k1) {
-          const RowSum<NR> g = gather(cidx[u].y);
-          lds[k + 1 - k0] = aval[u].y * g.v[0];
+          lds[k + 1 - k0] = aval[u].y * g1[u].v[0];
 #pragma unroll
-          for (int r = 1; r < NR; ++r) hold[r - 1][u][1] = aval[u].y * g.v[r];
+          for (int r = 1; r < NR; ++r) hold[r - 1][u][1] = aval[u].y * g1[u].v[r];
         }
       }
     }
@@ -370,19 +378,35 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
         aval[u] = load_vals2(P.val + k);
       }
     }
+    // Gathers in batches of kBatch steps, each batch issued whole before its products are
+    // added (out-of-segment lanes gather column 0 and add nothing), in the original order.
+    constexpr int kBatch = 5;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = kb + 2 * (u * kThreads + tid);
-      if (k < k1) {
-        if (k >= k0) {
-          const RowSum<NR> g = gather(cidx[u].x);
+    for (int ub = 0; ub < U; ub += kBatch) {
+      RowSum<NR> g0[kBatch], g1[kBatch];
 #pragma unroll
-          for (int r = 0; r < NR; ++r) seg.s[r] += aval[u].x * g.v[r];
+      for (int v = 0; v < kBatch; ++v) {
+        const int u = ub + v;
+        if (u < U) {
+          const int k = kb + 2 * (u * kThreads + tid);
+          g0[v] = gather((k < k1 && k >= k0) ? cidx[u].x : 0);
+          g1[v] = gather((k + 1 < k1) ? cidx[u].y : 0);
         }
-        if (k + 1 < k1) {
-          const RowSum<NR> g = gather(cidx[u].y);
+      }
 #pragma unroll
-          for (int r = 0; r < NR; ++r) seg.s[r] += aval[u].y * g.v[r];
+      for (int v = 0; v < kBatch; ++v) {
+        const int u = ub + v;
+        if (u >= U) continue;
+        const int k = kb + 2 * (u * kThreads + tid);
+        if (k < k1) {
+          if (k >= k0) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) seg.s[r] += aval[u].x * g0[v].v[r];
+          }
+          if (k + 1 < k1) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) seg.s[r] += aval[u].y * g1[v].v[r];
+          }
         }
       }
     }
