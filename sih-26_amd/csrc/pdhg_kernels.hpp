@@ -42,6 +42,9 @@ constexpr int kTile = SANKHYA_TILE;     // nonzeros per stream item
 constexpr int kSeg = SANKHYA_SEG;       // nonzeros per long-row segment
 constexpr int kTail = 1024;     // threads of the single-workgroup tail kernels
 constexpr int kChunk = 2048;    // long-row segment sums staged in LDS per tail round
+#ifndef SANKHYA_SEG_BATCH
+#define SANKHYA_SEG_BATCH 9  // long-row segment steps whose gathers are issued together
+#endif
 #ifndef SANKHYA_MAX_ROWS
 #define SANKHYA_MAX_ROWS 512
 #endif
@@ -380,7 +383,7 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
     }
     // Gathers in batches of kBatch steps, each batch issued whole before its products are
     // added (out-of-segment lanes gather column 0 and add nothing), in the original order.
-    constexpr int kBatch = 5;
+    constexpr int kBatch = SANKHYA_SEG_BATCH;
 #pragma unroll
     for (int ub = 0; ub < U; ub += kBatch) {
       RowSum<NR> g0[kBatch], g1[kBatch];
