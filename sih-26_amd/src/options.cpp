@@ -35,7 +35,9 @@ std::vector<OptionSpec> build_registry() {
   s.push_back({"gpu_devices", OptionType::String, std::string("auto"),
                "HIP device ID to run on; 'auto' is device 0. A list uses its first entry.", 0, 0, {}, "",
                true});
-  s.push_back({"threads", OptionType::Int, std::int64_t{1}, "Host worker threads.", 0.0, 1024.0});
+  s.push_back({"threads", OptionType::Int, std::int64_t{1},
+               "Host worker threads of the reference's CPU engine; accepted and not read here (the host "
+               "preparation of one large LP uses up to 16 threads).", 0.0, 1024.0});
   s.push_back({"deterministic", OptionType::Bool, false,
                "Reproducible solves. The MI355X engine uses no floating-point atomics, so its "
                "results repeat bit for bit and it is admitted under this option.", 0, 0});
