@@ -76,6 +76,7 @@ __device__ inline PlanView<V> lp_view(const Mat<V>& M, const PlanArrays& P, cons
   v.nitems = p.nitems;
   v.nlr = p.nlr;
   v.ngroups = p.ngrp;
+  v.has_group0 = 0;
   return v;
 }
 

@@ -414,6 +414,7 @@ struct DevPlan {
   int4* lr_info = nullptr;
   int4* lr_groups = nullptr;
   int32_t ngroups = 0;
+  int4 group0 = make_int4(0, 0, 0, 0);  // host copy of lr_groups[0]
   double* seg_part = nullptr;
   template <typename V>
   PlanView<V> view() const {
@@ -429,6 +430,8 @@ struct DevPlan {
     p.nitems = nitems;
     p.nlr = nlr;
     p.ngroups = ngroups;
+    p.group0 = group0;
+    p.has_group0 = ngroups > 0 ? 1 : 0;
     return p;
   }
 };
@@ -555,6 +558,7 @@ int make_plan(pdhg_dev* h, DevPlan* P, int32_t rows, int32_t ncols, int64_t nnz,
   if ((rc = upload(h, &P->lr_info, info.data(), info.size()))) return rc;
   if ((rc = upload(h, &P->lr_groups, hp.groups.data(), hp.groups.size()))) return rc;
   P->ngroups = static_cast<int32_t>(hp.groups.size());
+  if (!hp.groups.empty()) P->group0 = hp.groups[0];
   if ((rc = dalloc(h, &P->seg_part, static_cast<size_t>(hp.nseg) * 2))) return rc;
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return PDHG_OK;
