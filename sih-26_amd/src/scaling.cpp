@@ -58,7 +58,8 @@ Scaling build_scaling(const Model& model, const std::vector<double>& min_space_c
                                         std::vector<double>(um));
   // The Pock-Chambolle sums do depend on order, so that one pass stays a single sequential CSC
   // sweep, as in the reference.
-  const std::size_t chunk_cols = threads > 1 ? (un + threads - 1) / threads : un;
+  const std::size_t chunk_cols =
+      std::max<std::size_t>(1, threads > 1 ? (un + threads - 1) / threads : un);
   for (int pass = 0; pass <= ruiz_iterations; ++pass) {
     const bool pock_chambolle = pass == ruiz_iterations;
     if (threads == 1 || pock_chambolle) {
