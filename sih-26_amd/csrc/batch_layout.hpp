@@ -127,6 +127,7 @@ inline std::string build_batch_layout(int32_t nlp, const int32_t* m, const int32
     pg.max_rows = 2 * nt;
     pg.tail = nt;
     pg.bands = 1;
+    pg.threads = nt;
     for (int side = 0; side < 2; ++side) {
       PlanSet& ps = L->plans[side][g];
       int64_t src = 0, rp_off = 0;

@@ -33,8 +33,22 @@ struct HostPlan {
 // stream item, tail workgroup size (rows per tail group), and the number of column bands
 // (1 = no XCD banding, for plans that one workgroup walks alone).
 struct PlanGeo {
-  int tile = kTile, seg = kSeg, max_rows = kMaxRows, tail = kTail, bands = 8;
+  int tile = kTile, seg = kSeg, max_rows = kMaxRows, tail = kTail, bands = 8, threads = kThreads;
 };
+
+// A stream item's rows are summed by L lanes each, L the largest power of two (at most 64) with
+// rows * L <= threads; one lane then walks ceil(len / L) products of a row serially. An item is
+// closed before a row that would take that past kSerialSum (twice what a single full-tile row
+// costs at 256 threads), so a long row never lands among hundreds of tiny ones with one lane to
+// sum it: such an item, 300 empty rows and a 1,451-nonzero row, made a whole Zipf-LP launch 4x
+// slower. Plans whose rows come in length order (generated Zipf rows, uniform rows) never reach
+// the cap and are unchanged by it.
+constexpr int kSerialSum = 64;
+inline int lanes_per_row(int rows, int threads) {
+  int L = 64;
+  while (L > 1 && rows * L > threads) L >>= 1;
+  return L;
+}
 
 inline HostPlan build_plan(int32_t rows, int32_t ncols, const int32_t* rowptr, const int32_t* col,
                     std::vector<int4>* logical, const PlanGeo& g = PlanGeo{}) {
@@ -73,9 +87,14 @@ inline HostPlan build_plan(int32_t rows, int32_t ncols, const int32_t* rowptr, c
     }
     const int32_t r0 = r;
     int64_t acc = 0;
+    int32_t longest = 0;
     while (r < rows && r - r0 < g.max_rows) {
       const int32_t l = rowptr[r + 1] - rowptr[r];
       if (l > g.tile || acc + l > g.tile) break;
+      const int32_t lg = std::max(longest, l);
+      const int L = lanes_per_row(r - r0 + 1, g.threads);
+      if (r > r0 && (lg + L - 1) / L > kSerialSum) break;
+      longest = lg;
       acc += l;
       ++r;
     }

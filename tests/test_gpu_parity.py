@@ -91,6 +91,52 @@ def test_full_size_spmv_zipf_200k_500k():
     dev.close()
 
 
+def test_a_long_row_among_empty_rows_keeps_its_lanes():
+    """Rows ordered so that 300 empty rows precede each of 36 rows of 65 to 2,048 nonzeros. Packed
+    whole, such an item gives each row one lane, and that lane sums the long row alone (a 4x
+    slower dual product on the Zipf LP when it happened); the plan closes the item before the
+    long row instead. Products stay at parity with the oracle, and the dual kernel on this row
+    order takes less than twice its time on the generator's order."""
+    import scipy.sparse as sp
+    mdl, d = cm.generated("zipf", 20_000, 50_000, 10, 2)
+    m, n = d["m"], d["n"]
+    A = sp.csc_matrix((d["val"], d["rowidx"], d["colptr"]), shape=(m, n)).tocsr()
+    rl = np.diff(A.indptr)
+    empty = list(np.where(rl == 0)[0])
+    medium = [int(r) for r in np.where((rl > 64) & (rl <= 2048))[0]]
+    order, used = [], set()
+    for r in medium[: len(empty) // 300]:
+        block = empty[:300]
+        del empty[:300]
+        order += block + [r]
+        used.update(block + [r])
+    order += [r for r in range(m) if r not in used]
+    order = np.array(order)
+    assert sorted(order) == list(range(m)) and len(order) == m
+    B = A[order].tocsc()
+    B.sort_indices()
+    pm = sh.Model.from_csc(m, n, B.indptr, B.indices, B.data, d["cost"], d["col_lower"],
+                           d["col_upper"], d["row_lower"][order], d["row_upper"][order])
+    pd = pm.arrays()
+    P = orc.Problem(pd)
+    sc = orc.build_scaling(P)
+    times = []
+    for md in (mdl, pm):
+        dev = sh.Device(md)
+        if md is pm:
+            rng = np.random.default_rng(3)
+            x = rng.standard_normal(n)
+            ax = dev.spmv(x)
+            scale = orc.csc_multiply(P, np.abs(x), np.abs(sc["val"])) + 1e-300
+            assert np.max(np.abs(ax - orc.csc_multiply(P, x, sc["val"])) / scale) < 1e-13
+        dev.start(tolerance=1e-6, spectral_norm=1.0, stop_rule=1)
+        dev.run(1)
+        times.append(dev.time_kernels(20)[1])
+        dev.close()
+    print(f"k_dual: generator's row order {times[0]*1e3:.1f} us, interleaved {times[1]*1e3:.1f} us", flush=True)
+    assert times[1] < 2.0 * times[0], times
+
+
 @pytest.mark.parametrize("name,make,expected,tol", cm.KNOWN_ANSWERS, ids=[k[0] for k in cm.KNOWN_ANSWERS])
 def test_known_answers(name, make, expected, tol):
     s = sh.solve(cm.model_of(make()), pdhg_tolerance=1e-8, iteration_limit=200000, pdhg_polish=False)
