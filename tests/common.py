@@ -89,6 +89,30 @@ KNOWN_ANSWERS = [
 ]
 
 
+def kkt_instances():
+    """The 120 LPs of the reference's simplex-agreement test (tests/unit/test_pdhg.cpp:178-222,
+    generator tests/oracles/lp_generator.cpp:122-184, seed 20260904), as the fixture written by
+    tests/golden/kkt_instances.cpp: (lp dict, planted optimum). min c'x, A x >= b, 0 <= x <= u
+    (tests/oracles/rational_simplex.cpp:252-280)."""
+    with open(os.path.join(HERE, "golden", "kkt_20260904.json")) as fh:
+        raw = json.load(fh)
+    out = []
+    for lp in raw:
+        ub = [INF if u < 0 else float(u) for u in lp["upper"]]
+        out.append((dense_lp(lp["a"], lp["b"], [INF] * lp["rows"], lp["c"], col_upper=ub),
+                    float(lp["objective"])))
+    return out
+
+
+def large_rhs_lp():
+    """min x1 + x2, x1 + x2 >= 1e5: relative and absolute residuals five orders of magnitude
+    apart (tests/unit/test_pdhg.cpp:224-292)."""
+    return dense_lp([[1.0, 1.0]], [1.0e5], [INF], [1.0, 1.0])
+
+
+PRIMAL_FEASIBILITY = 1e-7  # include/sankhya/tolerances.hpp kPrimalFeasibility
+
+
 def c3_sizes(count: int, seed: int = 1000):
     """Row counts of the batched workload (sankhya_hip.workloads.batched_sizes)."""
     return wl.batched_sizes(count, seed)

@@ -58,6 +58,23 @@ def _check_against_oracle(data, sols, names, tol, stop_at_request):
         assert ok, (nm, checks)
 
 
+def test_batch_agrees_with_the_planted_optima_on_generated_instances():
+    """The 120 generated LPs of tests/unit/test_pdhg.cpp:178-222 (seed 20260904) as ONE batch at
+    1e-9: every LP that ends optimal is within 1e-6 relative of its planted optimum, and more
+    than 60 do."""
+    inst = cm.kkt_instances()
+    sols = sh.solve_batch([cm.model_of(d) for d, _ in inst], pdhg_tolerance=1e-9,
+                          iteration_limit=200000)
+    compared = 0
+    for (d, opt), s in zip(inst, sols):
+        if s.status != "optimal":
+            continue
+        compared += 1
+        assert abs(s.objective - opt) <= 1e-6 * max(1.0, abs(opt)), (d, s.objective, opt)
+    print(f"batch vs planted optimum: {compared} of {len(inst)} optimal", flush=True)
+    assert compared > 60
+
+
 @pytest.mark.parametrize("tune", [False, True])
 def test_batch_of_netlib_agrees_with_oracle_at_1e8(tune):
     names = ["afiro", "sc50a", "sc50b", "blend", "sc105", "adlittle", "stocfor1"]

@@ -137,6 +137,72 @@ def test_a_long_row_among_empty_rows_keeps_its_lanes():
     assert times[1] < 2.0 * times[0], times
 
 
+def test_agrees_with_the_planted_optima_on_generated_instances():
+    """tests/unit/test_pdhg.cpp:178-222: the 120 generated LPs of seed 20260904 solved at 1e-9
+    (polish off, 200,000 iterations); every one that ends optimal is within 1e-6 relative of
+    the optimum its construction plants (which HiGHS's simplex reaches too), and more than 60
+    end optimal."""
+    compared, worst = 0, 0.0
+    for d, opt in cm.kkt_instances():
+        s = sh.solve(cm.model_of(d), pdhg_tolerance=1e-9, iteration_limit=200000, pdhg_polish=False)
+        if s.status != "optimal":
+            continue
+        compared += 1
+        rel = abs(s.objective - opt) / max(1.0, abs(opt))
+        worst = max(worst, rel)
+        assert rel <= 1e-6, (d, s.objective, opt)
+    print(f"PDHG vs planted optimum: {compared} compared, worst relative difference {worst:.2e}", flush=True)
+    assert compared > 60
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e-6, 1e-8])
+def test_optimal_is_never_claimed_on_a_point_that_would_fail_verification(tol):
+    """tests/unit/test_pdhg.cpp:224-246: right-hand side 1e5."""
+    s = sh.solve(cm.model_of(cm.large_rhs_lp()), pdhg_tolerance=tol, iteration_limit=200000,
+                 pdhg_polish=False)
+    if s.status == "optimal":
+        assert s.primal_infeasibility <= cm.PRIMAL_FEASIBILITY, (tol, s.primal_infeasibility)
+
+
+def test_stops_only_on_a_point_that_meets_the_project_standard():
+    """tests/unit/test_pdhg.cpp:265-282: at 1e-4 the loop does not stop on a point the report
+    must then downgrade to feasible."""
+    s = sh.solve(cm.model_of(cm.large_rhs_lp()), pdhg_tolerance=1e-4, iteration_limit=200000,
+                 pdhg_polish=False)
+    assert s.status != "feasible", s.message
+    if s.status == "optimal":
+        assert s.primal_infeasibility <= cm.PRIMAL_FEASIBILITY
+
+
+def test_a_feasible_status_still_means_the_point_is_feasible():
+    """tests/unit/test_pdhg.cpp:284-292."""
+    s = sh.solve(cm.model_of(cm.large_rhs_lp()), pdhg_tolerance=1e-6, iteration_limit=200000,
+                 pdhg_polish=False)
+    if s.status in ("optimal", "feasible"):
+        assert s.primal_infeasibility <= cm.PRIMAL_FEASIBILITY
+
+
+def test_stop_at_request_gives_the_cheap_answer_and_never_calls_it_optimal():
+    """tests/unit/test_pdhg.cpp:340-382 on Netlib adlittle at a 1e-4 request: the default run
+    reaches optimal; with pdhg_stop_at_request the run stops earlier on a point reported
+    feasible, primal-feasible to 1e-7, at the default run's objective to 1e-3 relative, and the
+    message names the switch. Iteration counts are not pinned: the reference's comment quotes
+    192,080 and 132,520 for its CPU run, the oracle takes 209,160 and 72,400, and this engine
+    about 162,000 and 83,000; PDHG trajectories part at the rounding of the sums."""
+    mdl = cm.model_of(cm.netlib("adlittle"))
+    honest = sh.solve(mdl, pdhg_tolerance=1e-4, iteration_limit=1000000, pdhg_polish=False)
+    cheap = sh.solve(mdl, pdhg_tolerance=1e-4, iteration_limit=1000000, pdhg_polish=False,
+                     pdhg_stop_at_request=True)
+    print(f"adlittle: default {honest.status} at {honest.iterations} iterations, "
+          f"stop-at-request {cheap.status} at {cheap.iterations}", flush=True)
+    assert honest.status == "optimal", honest.message
+    assert cheap.status == "feasible", cheap.message
+    assert cheap.iterations < honest.iterations
+    assert cheap.primal_infeasibility <= cm.PRIMAL_FEASIBILITY, cheap.message
+    assert abs(cheap.objective - honest.objective) <= 1e-3 * max(1.0, abs(honest.objective))
+    assert "pdhg_stop_at_request" in cheap.message
+
+
 @pytest.mark.parametrize("name,make,expected,tol", cm.KNOWN_ANSWERS, ids=[k[0] for k in cm.KNOWN_ANSWERS])
 def test_known_answers(name, make, expected, tol):
     s = sh.solve(cm.model_of(make()), pdhg_tolerance=1e-8, iteration_limit=200000, pdhg_polish=False)

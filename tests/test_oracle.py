@@ -181,6 +181,39 @@ def test_oracle_known_answers(name, make, expected, tol):
     assert abs(r["objective"] - expected) <= tol
 
 
+def test_kkt_fixture_plants_the_optimum():
+    """The 120 generated instances (tests/golden/kkt_20260904.json): each planted point is
+    feasible, and HiGHS's dual simplex (scipy) reaches the planted objective exactly, so the
+    construction, not only the draw order, is right."""
+    from scipy.optimize import linprog
+    inst = cm.kkt_instances()
+    assert len(inst) == 120
+    dims = set()
+    for d, opt in inst:
+        P = orc.Problem(d)
+        A = np.zeros((P.m, P.n))
+        for j in range(P.n):
+            A[P.rowidx[P.colptr[j]:P.colptr[j + 1]], j] = P.val[P.colptr[j]:P.colptr[j + 1]]
+        dims.add((P.m, P.n))
+        r = linprog(P.cost, A_ub=-A, b_ub=-P.row_lower,
+                    bounds=[(0, None if math.isinf(u) else u) for u in P.col_upper], method="highs-ds")
+        assert r.status == 0 and abs(r.fun - opt) <= 1e-9 * max(1.0, abs(opt)), (r.fun, opt)
+    assert min(dims) >= (2, 2) and max(m for m, _ in dims) <= 6 and max(n for _, n in dims) <= 6
+
+
+def test_oracle_agrees_with_the_planted_optima():
+    """tests/unit/test_pdhg.cpp:178-222 on the oracle: at 1e-9, every instance that ends
+    optimal is within 1e-6 relative of the optimum, and more than 60 do."""
+    compared = 0
+    for d, opt in cm.kkt_instances():
+        r = orc.solve(orc.Problem(d), tolerance=1e-9, iteration_limit=200000)
+        if r["status"] != "optimal":
+            continue
+        compared += 1
+        assert abs(r["objective"] - opt) <= 1e-6 * max(1.0, abs(opt)), (r["objective"], opt)
+    assert compared > 60
+
+
 def test_oracle_iteration_limit_and_first_step_guard():
     P = orc.Problem(cm.dense_lp([[1, 1]], [2], [np.inf], [1, 1]))
     r = orc.solve(P, tolerance=1e-12, iteration_limit=20)
