@@ -233,7 +233,22 @@ def _compare(d, s, r, tol=1e-6):
     return rel
 
 
-NETLIB_FAST = ["afiro", "sc50a", "sc50b", "blend", "sc105", "adlittle", "stocfor1"]
+NETLIB_FAST = ["afiro", "sc50a", "sc50b", "blend", "sc105", "adlittle", "stocfor1", "israel"]
+
+
+def test_netlib_share2b_at_the_iteration_limit():
+    """The ninth instance of tests/unit/test_pdhg_cuda_regression.cpp:73-75. At 1e-8 within
+    1,000,000 iterations the oracle stops at the iteration limit (so the reference test's
+    'both optimal' is not what its CPU engine does either); the engine ends with the oracle's
+    status, both within 1e-6 of the published optimum."""
+    d = cm.netlib("share2b")
+    s = sh.solve(cm.model_of(d), pdhg_tolerance=1e-8, iteration_limit=1000000, pdhg_polish=False)
+    r = orc.solve(orc.Problem(d), tolerance=1e-8, iteration_limit=1000000)
+    pub = cm.netlib_optima()["share2b"]["published_optimal"]
+    print(f"share2b: gpu {s.status} {s.objective:.12g} it={s.iterations}; oracle {r['status']} "
+          f"{r['objective']:.12g} it={r['iterations']}", flush=True)
+    assert s.status == r["status"] == "iteration_limit"
+    assert abs(s.objective - pub) <= 1e-6 * abs(pub) and abs(r["objective"] - pub) <= 1e-6 * abs(pub)
 
 
 @pytest.mark.parametrize("name", NETLIB_FAST)
