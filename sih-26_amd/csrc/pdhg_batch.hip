@@ -921,14 +921,18 @@ int bfail(pdhg_batch* h, int code, const std::string& what) {
 template <typename T>
 int balloc(pdhg_batch* h, T** p, size_t count) {
   *p = nullptr;
-  if (count == 0) count = 1;
-  const size_t bytes = count * sizeof(T);
+  // kSpare zeroed elements past the end: a lane outside its item gathers index 0 of its LP, which
+  // for an LP with no rows (or no columns) is the next LP's first entry, or, for the last LP of
+  // the batch, one past the array.
+  constexpr size_t kSpare = 2;
+  const size_t bytes = (count + kSpare) * sizeof(T);
   if (hipMalloc(reinterpret_cast<void**>(p), bytes) != hipSuccess) {
     *p = nullptr;
     return bfail(h, PDHG_E_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
   }
   h->allocs.push_back(*p);
   h->bytes += static_cast<int64_t>(bytes);
+  BHIP_TRY(h, hipMemsetAsync(*p + count, 0, kSpare * sizeof(T), h->stream));
   return PDHG_OK;
 }
 

@@ -495,17 +495,23 @@ bool injected(pdhg_dev* h) {
   return false;
 }
 
+// Every device array carries kSpare zeroed elements past its end. A lane outside its item
+// gathers index 0 (pair element `cur` for the iterate pairs) and discards it; when the gathered
+// vector is empty (the transpose of an LP with no rows, or an LP with no columns) that index
+// lies past the vector's data, and the spare elements keep the read inside the allocation.
+constexpr size_t kSpare = 2;
+
 template <typename T>
 int dalloc(pdhg_dev* h, T** p, size_t count) {
   *p = nullptr;
-  if (count == 0) count = 1;
-  const size_t bytes = count * sizeof(T);
+  const size_t bytes = (count + kSpare) * sizeof(T);
   if (hipMalloc(reinterpret_cast<void**>(p), bytes) != hipSuccess) {
     *p = nullptr;
     return fail(h, PDHG_E_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
   }
   h->allocs.push_back(*p);
   h->bytes += static_cast<int64_t>(bytes);
+  HIP_TRY(h, hipMemsetAsync(*p + count, 0, kSpare * sizeof(T), h->stream));
   return PDHG_OK;
 }
 

@@ -113,6 +113,44 @@ def large_rhs_lp():
 PRIMAL_FEASIBILITY = 1e-7  # include/sankhya/tolerances.hpp kPrimalFeasibility
 
 
+def sparse_lp(m, n, cols, cost, col_lower, col_upper, row_lower, row_upper):
+    """An LP from {column: [(row, value), ...]}, min sense."""
+    colptr, rowidx, val = [0], [], []
+    for j in range(n):
+        for i, a in cols.get(j, []):
+            rowidx.append(i)
+            val.append(a)
+        colptr.append(len(rowidx))
+    return dict(m=m, n=n, colptr=np.array(colptr, np.int32), rowidx=np.array(rowidx, np.int32),
+                val=np.array(val, float), cost=np.array(cost, float),
+                col_lower=np.array(col_lower, float), col_upper=np.array(col_upper, float),
+                row_lower=np.array(row_lower, float), row_upper=np.array(row_upper, float),
+                objective_offset=0.0, maximize=False)
+
+
+# Degenerate shapes: (name, lp, expected status, expected objective or None). The optima are
+# by hand; PDHG never reports infeasibility (src/pdhg/pdhg.cpp:486-567), so an infeasible LP
+# ends at the iteration limit.
+DEGENERATE = [
+    # min x0 - 2 x1 + x2/2 over a box: x = (0, 5, -1)
+    ("no_rows", sparse_lp(0, 3, {}, [1, -2, 0.5], [0, 0, -1], [4, 5, 3], [], []), "optimal", -10.5),
+    # -1 <= 0 <= 1 and 0 = 0
+    ("rows_without_entries", sparse_lp(2, 2, {}, [1, 1], [0, 0], [3, 3], [-1, 0], [1, 0]),
+     "optimal", 0.0),
+    # column 1 has no entries: x1 at its lower bound 1; x0 + x2 >= 4 at x0 = 4
+    ("empty_column", sparse_lp(1, 3, {0: [(0, 1.0)], 2: [(0, 1.0)]}, [1, 2, 3], [0, 1, 0],
+                               [10, 10, 10], [4], [INF]), "optimal", 6.0),
+    # x0 fixed at 2; x0 + x1 >= 3 and x0 - x1 <= 5: x1 = 1. (A free row, both bounds infinite,
+    # is left out: its complementarity term is |y| * inf (src/pdhg/pdhg_evaluate.cpp:103-114),
+    # so the stopping test holds only at a tick where that row's y is exactly zero.)
+    ("fixed_column", sparse_lp(2, 2, {0: [(0, 1.0), (1, 1.0)], 1: [(0, 1.0), (1, -1.0)]},
+                               [1, 1], [2, 0], [2, 10], [3, -INF], [INF, 5]),
+     "optimal", 3.0),
+    # x0 <= 1 and x0 >= 5
+    ("infeasible", sparse_lp(1, 1, {0: [(0, 1.0)]}, [1], [0], [1], [5], [INF]), "iteration_limit", None),
+]
+
+
 def c3_sizes(count: int, seed: int = 1000):
     """Row counts of the batched workload (sankhya_hip.workloads.batched_sizes)."""
     return wl.batched_sizes(count, seed)

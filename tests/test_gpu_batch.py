@@ -58,6 +58,19 @@ def _check_against_oracle(data, sols, names, tol, stop_at_request):
         assert ok, (nm, checks)
 
 
+def test_batch_of_degenerate_shapes():
+    """The degenerate LPs in one batch, the LP with no rows last (its gathers then address the
+    end of the concatenated arrays): each ends with its expected status and objective."""
+    cases = [c for c in cm.DEGENERATE if c[0] != "no_rows"] + [c for c in cm.DEGENERATE if c[0] == "no_rows"]
+    sols = sh.solve_batch([cm.model_of(d) for _, d, _, _ in cases], pdhg_tolerance=1e-8,
+                          iteration_limit=100000)
+    for (name, d, status, objective), s in zip(cases, sols):
+        assert s.status == status, (name, s.status, s.message)
+        if objective is not None:
+            assert abs(s.objective - objective) <= 1e-6 * max(1.0, abs(objective)), (name, s.objective)
+        assert np.all(np.isfinite(s.col_value)), name
+
+
 def test_batch_agrees_with_the_planted_optima_on_generated_instances():
     """The 120 generated LPs of tests/unit/test_pdhg.cpp:178-222 (seed 20260904) as ONE batch at
     1e-9: every LP that ends optimal is within 1e-6 relative of its planted optimum, and more

@@ -203,6 +203,19 @@ def test_stop_at_request_gives_the_cheap_answer_and_never_calls_it_optimal():
     assert "pdhg_stop_at_request" in cheap.message
 
 
+@pytest.mark.parametrize("name,d,status,objective", cm.DEGENERATE, ids=[c[0] for c in cm.DEGENERATE])
+def test_degenerate_shapes_agree_with_oracle(name, d, status, objective):
+    """No rows, rows without entries, an empty column, a fixed column, and an infeasible LP: the engine ends with the oracle's status and objective, and a claimed point
+    passes the acceptance check."""
+    s = sh.solve(cm.model_of(d), pdhg_tolerance=1e-8, iteration_limit=100000, pdhg_polish=False)
+    r = orc.solve(orc.Problem(d), tolerance=1e-8, iteration_limit=100000)
+    assert s.status == r["status"] == status, (s.status, r["status"], s.message)
+    if objective is not None:
+        assert abs(s.objective - objective) <= 1e-6 * max(1.0, abs(objective))
+        _compare(d, s, r)
+    assert np.all(np.isfinite(s.col_value))
+
+
 @pytest.mark.parametrize("name,make,expected,tol", cm.KNOWN_ANSWERS, ids=[k[0] for k in cm.KNOWN_ANSWERS])
 def test_known_answers(name, make, expected, tol):
     s = sh.solve(cm.model_of(make()), pdhg_tolerance=1e-8, iteration_limit=200000, pdhg_polish=False)

@@ -214,6 +214,14 @@ def test_oracle_agrees_with_the_planted_optima():
     assert compared > 60
 
 
+@pytest.mark.parametrize("name,d,status,objective", cm.DEGENERATE, ids=[c[0] for c in cm.DEGENERATE])
+def test_oracle_degenerate_shapes(name, d, status, objective):
+    r = orc.solve(orc.Problem(d), tolerance=1e-8, iteration_limit=100000)
+    assert r["status"] == status
+    if objective is not None:
+        assert abs(r["objective"] - objective) <= 1e-6 * max(1.0, abs(objective))
+
+
 def test_oracle_iteration_limit_and_first_step_guard():
     P = orc.Problem(cm.dense_lp([[1, 1]], [2], [np.inf], [1, 1]))
     r = orc.solve(P, tolerance=1e-12, iteration_limit=20)
