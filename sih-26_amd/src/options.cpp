@@ -40,7 +40,8 @@ std::vector<OptionSpec> build_registry() {
                "preparation of one large LP uses up to 16 threads).", 0.0, 1024.0});
   s.push_back({"deterministic", OptionType::Bool, false,
                "Reproducible solves. The MI355X engine uses no floating-point atomics, so its "
-               "results repeat bit for bit and it is admitted under this option.", 0, 0});
+               "results repeat bit for bit and it is admitted under this option; the batched "
+               "engine then skips its timing-based schedule tuner.", 0, 0});
   s.push_back({"random_seed", OptionType::Int, std::int64_t{0}, "Seed; PDHG's power iteration uses seed + 1.",
                0.0, 4294967295.0});
   s.push_back({"pdhg_restart", OptionType::Bool, true, "Restart PDHG on the KKT-error criterion.", 0, 0});

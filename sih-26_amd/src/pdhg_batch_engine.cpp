@@ -125,7 +125,11 @@ std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models
   logger.info("Solving {} LPs with MI355X batched PDHG on {}, precision {}", nb, desc,
               mixed ? "mixed" : "fp64");
   std::vector<int32_t> geo(nb, -1);
-  if (options.get_bool("pdhg_batch_tune")) {
+  // The tuner decides from measured times, so a second run may give an LP another schedule,
+  // and with it another order of its partial sums. Under `deterministic` every LP stays on
+  // the cooperative schedule and the results repeat bit for bit.
+  const bool tune = options.get_bool("pdhg_batch_tune") && !options.get_bool("deterministic");
+  if (tune) {
     rc = pdhg_batch_tune(dev_h, &params, norms.data(),
                          static_cast<int>(options.get_int("pdhg_batch_tune_attempts")), geo.data());
     if (rc != PDHG_OK) return dev_error(rc, "schedule tuning");

@@ -11,6 +11,7 @@ single-LP engine, per LP:
     within 1e-5 relative: both are 1e-6-accurate points);
   - the schedule tuner returns a geometry for every LP and does not change answers.
 """
+import math
 import time
 
 import numpy as np
@@ -56,6 +57,21 @@ def _check_against_oracle(data, sols, names, tol, stop_at_request):
         assert rel <= 1e-6, (nm, s.objective, r["objective"])
         ok, checks = accepted(d, _sol(s))
         assert ok, (nm, checks)
+
+
+def test_deterministic_batches_repeat_bit_for_bit():
+    """tests/unit/test_deterministic.cpp:187-200 for the batched engine: with deterministic=true
+    (which turns the timing-based tuner off) two runs of the same batch return identical bits,
+    and the single-LP engine repeats itself likewise."""
+    models = [cm.model_of(cm.netlib(nm)) for nm in ("afiro", "sc50a", "blend")]
+    models += [cm.generated("uniform", m, 3 * m, 5, 40 + k)[0] for k, m in enumerate((120, 300))]
+    runs = [sh.solve_batch(models, pdhg_tolerance=1e-8, iteration_limit=200000, deterministic=True)
+            for _ in range(2)]
+    for a, b in zip(*runs):
+        assert a.status == b.status and a.iterations == b.iterations
+        assert a.objective == b.objective or (math.isnan(a.objective) and math.isnan(b.objective))
+        assert np.array_equal(a.col_value, b.col_value) and np.array_equal(a.row_dual, b.row_dual)
+        assert a.stats["schedule"] == b.stats["schedule"] == -1
 
 
 def test_batch_of_degenerate_shapes():
