@@ -140,6 +140,20 @@ def test_model_errors_are_reported_before_any_device_work():
     d = cm.dense_lp([[1, 1]], [5], [2], [1, 1])
     s = sh.solve(cm.model_of(d))
     assert s.status == "model_error" and "above upper bound" in s.message
+    for bad in (np.nan, np.inf):
+        d = cm.dense_lp([[1, 1]], [2], [math.inf], [1, 1])
+        d["val"] = np.array([bad, 1.0])
+        s = sh.solve(cm.model_of(d))
+        assert s.status == "model_error" and "not a usable coefficient" in s.message
+
+
+@pytest.mark.parametrize("colptr,rowidx", [([0, 1, 2], [0, 5]), ([0, 1, 2], [0, -1]),
+                                           ([0, 2, 1], [0, 1]), ([0, 2, 2], [0, 0])],
+                         ids=["row_out_of_range", "negative_row", "colptr_not_monotone",
+                              "duplicate_entry"])
+def test_malformed_matrices_are_refused_at_construction(colptr, rowidx):
+    with pytest.raises(ValueError, match="model construction failed"):
+        sh.Model.from_csc(2, 2, colptr, rowidx, [1.0, 1.0], [1, 1], [0, 0], [1, 1], [0, 0], [1, 1])
 
 
 def test_no_device_is_an_error_not_a_cpu_fallback():
