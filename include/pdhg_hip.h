@@ -65,6 +65,26 @@ int pdhg_dev_upload(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz, const int32_
                     const double* col_lower, const double* col_upper, const double* row_lower,
                     const double* row_upper, double bound_norm, double cost_norm);
 
+/* The same upload from the UNSCALED problem, preconditioned on the device: `ruiz_iterations`
+ * Ruiz rounds and one Pock-Chambolle round (src/la/scaling.cpp:14-159, build_scaling :111-159),
+ * bit-identical to the reference's host passes, applied in place to both copies of the matrix,
+ * then the scaled cost and bounds. a_val / at_val hold A's values in CSR / CSC order; min_cost
+ * is sense * c; the norms are of the unscaled problem (pdhg.cpp:94-111). extremes (may be
+ * null) receives the smallest and largest |entry| of the scaled matrix. */
+int pdhg_dev_upload_unscaled(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz,
+                             const int32_t* a_rowptr, const int32_t* a_colidx, const double* a_val,
+                             const int32_t* at_rowptr, const int32_t* at_colidx,
+                             const double* at_val, const double* min_cost, const double* col_lower,
+                             const double* col_upper, const double* row_lower,
+                             const double* row_upper, double bound_norm, double cost_norm,
+                             int ruiz_iterations, double* extremes);
+
+/* Test hook: the device's preconditioning results, Dr (m), Dc (n), the scaled values in CSC
+ * order (nnz) and the scaled cost and bounds; any pointer may be null. */
+int pdhg_dev_scaling(pdhg_dev* h, double* row, double* column, double* csc_values, double* cost_hat,
+                     double* col_lower_hat, double* col_upper_hat, double* row_lower_hat,
+                     double* row_upper_hat);
+
 /* Replaces estimate_spectral_norm (src/la/scaling.cpp:161-194) on the device: `iterations`
  * power steps on Ahat^T Ahat from the start vector v0 (n, host), rounded up by 1.01. */
 int pdhg_dev_spectral_norm(pdhg_dev* h, const double* v0, int iterations, double* norm);

@@ -122,8 +122,16 @@ int sankhya_pdhg_scaling(const sankhya_model* model, int ruiz_iterations, int th
                          double* col_lower_hat, double* col_upper_hat, double* row_lower_hat,
                          double* row_upper_hat);
 
-/* Opens a device handle holding this model exactly as the engine uploads it (scaled, both
- * CSR layouts), for the pdhg_dev_* test hooks. Also returns the unscaled min-space cost norm
+/* The CSR of the scaled matrix Ahat as the host preparation builds it (host preconditioner,
+ * then the counting-sort transpose of src/la/sparse.cpp:271-302, CsrView::build :315-322):
+ * rowptr (m + 1), colidx and values (nnz), columns ascending within each row. threads: host
+ * threads for the scaling and the transpose (the same arrays for any count). The batched
+ * engine uploads this; the single-LP engine uploads A's own CSR and scales on the device. */
+int sankhya_pdhg_csr(const sankhya_model* model, int threads, int32_t* rowptr, int32_t* colidx,
+                     double* values);
+
+/* Opens a device handle holding this model exactly as the engine uploads it (both CSR layouts,
+ * preconditioned on the device by pdhg_dev_upload_unscaled), for the pdhg_dev_* test hooks. Also returns the unscaled min-space cost norm
  * and bound norm through the handle. */
 int sankhya_pdhg_device_open(const sankhya_model* model, int device, int precision,
                              pdhg_dev** out);

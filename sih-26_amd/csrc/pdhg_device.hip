@@ -402,6 +402,240 @@ __global__ __launch_bounds__(kThreads) void k_pw_finish(Ctl* ctl, const double* 
   if (next == 0.0) ctl->pw_zero = 1;
 }
 
+// ---- preconditioning on the device (src/la/scaling.cpp:14-159) ---------------------------
+// The same passes as the host preconditioner, bit for bit: a Ruiz round takes each row's and
+// column's largest |a| (order-free), forms r = 1/sqrt(max) (1 for an empty line) and scales
+// every entry as (a * r_i) * c_j in both stored copies; the Pock-Chambolle round sums |a| per
+// row in ascending column order and per column in ascending row order, one lane per line,
+// which is the order in which the reference's column sweep adds them.
+
+// The kernels walk a matrix's work plan (the same items the products use): a stream item's
+// rows one wave each, a long row's segments (at most kSeg entries) one workgroup each, so a
+// dense row of hundreds of thousands of entries is spread over many workgroups. kScU coalesced
+// loads per lane are in flight per step.
+constexpr int kScU = 8;
+
+// Largest |a| of entries [k0, k1), one wave; every lane returns it.
+__device__ __forceinline__ double wave_absmax(const double* val, int32_t k0, int32_t k1, int lane) {
+  double r = 0.0;
+  for (int32_t base = k0; base < k1; base += 64 * kScU) {
+    double v[kScU];
+#pragma unroll
+    for (int u = 0; u < kScU; ++u) {
+      const int32_t k = base + u * 64 + lane;
+      v[u] = k < k1 ? val[k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kScU; ++u) r = smax(r, fabs(v[u]));
+  }
+  for (int off = 32; off > 0; off >>= 1) r = smax(r, __shfl_xor(r, off));
+  return r;
+}
+
+// Entries [k0, k1) become (a * r_i) * c_j, one wave.
+__device__ __forceinline__ void wave_apply(const int32_t* idx, double* val, int32_t k0, int32_t k1,
+                                           int lane, double lf, const double* idx_pass,
+                                           int line_is_row) {
+  for (int32_t base = k0; base < k1; base += 64 * kScU) {
+    int32_t id[kScU];
+    double v[kScU], other[kScU];
+#pragma unroll
+    for (int u = 0; u < kScU; ++u) {
+      const int32_t k = base + u * 64 + lane;
+      id[u] = k < k1 ? idx[k] : 0;  // index 0 exists: this line has an entry
+      v[u] = k < k1 ? val[k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kScU; ++u) other[u] = idx_pass[id[u]];
+#pragma unroll
+    for (int u = 0; u < kScU; ++u) {
+      const int32_t k = base + u * 64 + lane;
+      const double r = line_is_row ? lf : other[u], c = line_is_row ? other[u] : lf;
+      if (k < k1) val[k] = v[u] * r * c;
+    }
+  }
+}
+
+// Sum of |a| over [k0, k1) in storage order, one wave: the wave stages a step of entries in
+// its LDS slice with coalesced loads, then lane 0 adds them one by one in order, so the result
+// is the sequential sum bit for bit. Only lane 0's result is meaningful.
+__device__ __forceinline__ double wave_abssum_ordered(const double* val, int32_t k0, int32_t k1,
+                                                      int lane, double* lds /* 64 kScU */) {
+  double r = 0.0;
+  for (int32_t base = k0; base < k1; base += 64 * kScU) {
+    double v[kScU];
+#pragma unroll
+    for (int u = 0; u < kScU; ++u) {
+      const int32_t k = base + u * 64 + lane;
+      v[u] = k < k1 ? fabs(val[k]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kScU; ++u) lds[u * 64 + lane] = v[u];
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+      const int32_t count = k1 - base < 64 * kScU ? k1 - base : 64 * kScU;
+      for (int32_t e = 0; e < count; ++e) r += lds[e];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  return r;
+}
+
+// One workgroup per plan item. mode 0: largest |a| per row of a stream item into stat, per
+// segment into seg (a long row's segments are combined by k_sc_longmax); mode 1: scale the
+// item's entries. For the CSR the line factor is the row's, for the CSC the column's.
+__global__ __launch_bounds__(kThreads) void k_sc_items(const int4* items, const int32_t* ptr,
+                                                       const int32_t* idx, double* val,
+                                                       const int4* lr_info, int mode, double* stat,
+                                                       double* seg, const double* line_pass,
+                                                       const double* idx_pass, int line_is_row) {
+  __shared__ double red[kThreads / 64];
+  const int4 it = items[blockIdx.x];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (it.y >= 0) {
+    for (int32_t row = it.x + wave; row < it.y; row += kThreads / 64) {
+      const int32_t k0 = ptr[row], k1 = ptr[row + 1];
+      if (mode == 0) {
+        const double r = wave_absmax(val, k0, k1, lane);
+        if (lane == 0) stat[row] = r;
+      } else {
+        wave_apply(idx, val, k0, k1, lane, line_pass[row], idx_pass, line_is_row);
+      }
+    }
+    return;
+  }
+  // A segment: the workgroup's four waves take quarters of its entries.
+  const int32_t k0 = it.z, k1 = it.w;
+  const int32_t quarter = (k1 - k0 + kThreads / 64 - 1) / (kThreads / 64);
+  const int32_t q0 = k0 + wave * quarter, q1 = q0 + quarter < k1 ? q0 + quarter : k1;
+  if (mode == 0) {
+    const double r = q0 < q1 ? wave_absmax(val, q0, q1, lane) : 0.0;
+    if (lane == 0) red[wave] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = red[0];
+      for (int w = 1; w < kThreads / 64; ++w) m = smax(m, red[w]);
+      seg[it.x] = m;
+    }
+  } else if (q0 < q1) {
+    const int32_t row = lr_info[-it.y - 1].x;
+    wave_apply(idx, val, q0, q1, lane, line_pass[row], idx_pass, line_is_row);
+  }
+}
+
+// A long row's largest |a| from its segments' maxima (lr_info: row, first segment, count).
+__global__ __launch_bounds__(kThreads) void k_sc_longmax(const int4* lr_info, int32_t nlr,
+                                                         const double* seg, double* stat) {
+  const int32_t lr = static_cast<int32_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (lr >= nlr) return;
+  const int4 info = lr_info[lr];
+  double m = 0.0;
+  for (int32_t q = 0; q < info.z; ++q) m = smax(m, seg[info.y + q]);
+  stat[info.x] = m;
+}
+
+// The Pock-Chambolle 1-norm of every line, one wave per line in storage order (a long row's
+// entries too: its sum is one sequential chain).
+__global__ __launch_bounds__(kThreads) void k_sc_linesum(const int32_t* ptr, const double* val,
+                                                         int32_t lines, double* stat) {
+  __shared__ double stage[kThreads * kScU];
+  const int lane = threadIdx.x & 63;
+  const int64_t line = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
+  if (line >= lines) return;  // the whole wave leaves together
+  const double r = wave_abssum_ordered(val, ptr[line], ptr[line + 1], lane,
+                                       stage + (threadIdx.x >> 6) * 64 * kScU);
+  if (lane == 0) stat[line] = r;
+}
+
+// This round's factor 1/sqrt(stat) (1 for an empty line) and the running product d *= factor.
+__global__ __launch_bounds__(kThreads) void k_sc_factor(const double* stat, double* pass, double* d,
+                                                        int32_t count) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < count;
+       i += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const double s = stat[i];
+    const double f = s > 0.0 ? 1.0 / sqrt(s) : 1.0;
+    pass[i] = f;
+    d[i] *= f;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_sc_fill(double* v, int32_t count, double value) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < count;
+       i += static_cast<int64_t>(gridDim.x) * kThreads)
+    v[i] = value;
+}
+
+// Scaled cost and bounds: c^ = c Dc, l^ = l (1/Dc), u^ = u (1/Dc), rl^ = rl Dr, ru^ = ru Dr;
+// infinite sides stay infinite (scaling.cpp:105-107,139-157).
+__global__ __launch_bounds__(kThreads) void k_sc_vectors(int32_t n, int32_t m, const double* dc,
+                                                         const double* dr, const double* cost,
+                                                         const double* cl, const double* cu,
+                                                         const double* rl, const double* ru,
+                                                         double* chat, double* lhat, double* uhat,
+                                                         double* rlhat, double* ruhat) {
+  const int64_t most = n > m ? n : m;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < most;
+       i += static_cast<int64_t>(gridDim.x) * kThreads) {
+    if (i < n) {
+      const double d = dc[i], inv = 1.0 / d;
+      chat[i] = cost[i] * d;
+      lhat[i] = finite_bound(cl[i]) ? cl[i] * inv : cl[i];
+      uhat[i] = finite_bound(cu[i]) ? cu[i] * inv : cu[i];
+    }
+    if (i < m) {
+      const double d = dr[i];
+      rlhat[i] = finite_bound(rl[i]) ? rl[i] * d : rl[i];
+      ruhat[i] = finite_bound(ru[i]) ? ru[i] * d : ru[i];
+    }
+  }
+}
+
+// Smallest and largest |a| of the scaled matrix: per-workgroup extremes into part[2 g], then
+// one workgroup combines them into out[0..1] (minimum and maximum are exact in any order).
+__global__ __launch_bounds__(kThreads) void k_sc_extremes(const double* val, int64_t nnz,
+                                                          double* part) {
+  __shared__ double lo[kThreads], hi[kThreads];
+  double a = nnz > 0 ? fabs(val[0]) : 0.0, z = 0.0;
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; k < nnz;
+       k += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const double mag = fabs(val[k]);
+    a = smin(a, mag);
+    z = smax(z, mag);
+  }
+  lo[threadIdx.x] = a;
+  hi[threadIdx.x] = z;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) {
+      lo[threadIdx.x] = smin(lo[threadIdx.x], lo[threadIdx.x + w]);
+      hi[threadIdx.x] = smax(hi[threadIdx.x], hi[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = lo[0];
+    part[2 * blockIdx.x + 1] = hi[0];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_sc_extremes_final(const double* part, int blocks,
+                                                                double* out) {
+  if (threadIdx.x != 0) return;
+  double a = part[0], z = part[1];
+  for (int b = 1; b < blocks; ++b) {
+    a = smin(a, part[2 * b]);
+    z = smax(z, part[2 * b + 1]);
+  }
+  out[0] = a;
+  out[1] = z;
+}
+
+__global__ __launch_bounds__(kThreads) void k_sc_narrow(const double* v, float* f, int64_t count) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; k < count;
+       k += static_cast<int64_t>(gridDim.x) * kThreads)
+    f[k] = static_cast<float>(v[k]);
+}
+
 // ---- host side --------------------------------------------------------------------------
 
 struct DevPlan {
@@ -535,28 +769,18 @@ int make_plan(pdhg_dev* h, DevPlan* P, int32_t rows, int32_t ncols, int64_t nnz,
   P->nseg = hp.nseg;
   int rc;
   if ((rc = upload(h, &P->rowptr, rowptr, static_cast<size_t>(rows) + 1))) return rc;
-  // Index and value arrays carry kPad zeroed entries past nnz: the kernels load nonzeros in
-  // aligned pairs, so the last pair of an odd-length array reads one entry past the end.
-  constexpr size_t kPad = 2;
+  // The kernels load nonzeros in aligned pairs, so the last pair of an odd-length array reads
+  // one entry past nnz: the allocator's kSpare zeroed elements are that pad.
+  static_assert(kSpare >= 2, "pair loads need a zeroed pad");
   const size_t un = static_cast<size_t>(nnz);
-  {
-    std::vector<int32_t> c(un + kPad, 0);
-    std::copy(col, col + nnz, c.begin());
-    if ((rc = upload(h, &P->col, c.data(), c.size()))) return rc;
-    HIP_TRY(h, hipStreamSynchronize(h->stream));
-  }
+  if ((rc = upload(h, &P->col, col, un))) return rc;
   if (h->precision == PDHG_PRECISION_MIXED) {
-    std::vector<float> f(un + kPad, 0.0f);
+    std::vector<float> f(un);
     for (int64_t k = 0; k < nnz; ++k) f[k] = static_cast<float>(val[k]);
     if ((rc = upload(h, &P->val32, f.data(), f.size()))) return rc;
     HIP_TRY(h, hipStreamSynchronize(h->stream));
   }
-  {
-    std::vector<double> v(un + kPad, 0.0);
-    std::copy(val, val + nnz, v.begin());
-    if ((rc = upload(h, &P->val64, v.data(), v.size()))) return rc;
-    HIP_TRY(h, hipStreamSynchronize(h->stream));
-  }
+  if ((rc = upload(h, &P->val64, val, un))) return rc;
   if ((rc = upload(h, &P->items, hp.items.data(), hp.items.size()))) return rc;
   std::vector<int4> info(hp.lr_nseg.size());
   for (size_t k = 0; k < info.size(); ++k)
@@ -714,6 +938,67 @@ int check_ready(pdhg_dev* h, bool need_start) {
   return PDHG_OK;
 }
 
+// Ruiz rounds and one Pock-Chambolle round on the uploaded (unscaled) matrix, in place in
+// both copies, then the scaled cost and bounds (and the FP32 copies in mixed precision).
+// Scratch: sy and sact hold the row statistics and factors, sx and sx2 the column ones.
+int device_scale(pdhg_dev* h, int ruiz_iterations, double* extremes) {
+  const int32_t m = h->m, n = h->n;
+  double *rstat = h->sy, *rpass = h->sact, *cstat = h->sx, *cpass = h->sx2;
+  auto waves = [](int32_t lines) { return static_cast<unsigned>((lines + kThreads / 64 - 1) / (kThreads / 64)); };
+  auto lanes = [](int64_t count) {
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((count + kThreads - 1) / kThreads, 8192)));
+  };
+  if (m > 0) k_sc_fill<<<lanes(m), kThreads, 0, h->stream>>>(h->dr, m, 1.0);
+  if (n > 0) k_sc_fill<<<lanes(n), kThreads, 0, h->stream>>>(h->dc, n, 1.0);
+  auto items = [&](const DevPlan& P, int mode, double* stat, const double* line_pass,
+                   const double* idx_pass, int line_is_row) {
+    if (P.nitems > 0)
+      k_sc_items<<<P.nitems, kThreads, 0, h->stream>>>(P.items, P.rowptr, P.col, P.val64, P.lr_info,
+                                                       mode, stat, P.seg_part, line_pass, idx_pass,
+                                                       line_is_row);
+    if (mode == 0 && P.nlr > 0)
+      k_sc_longmax<<<(P.nlr + kThreads - 1) / kThreads, kThreads, 0, h->stream>>>(P.lr_info, P.nlr,
+                                                                                  P.seg_part, stat);
+  };
+  auto factors_and_apply = [&]() {
+    if (m > 0) k_sc_factor<<<lanes(m), kThreads, 0, h->stream>>>(rstat, rpass, h->dr, m);
+    if (n > 0) k_sc_factor<<<lanes(n), kThreads, 0, h->stream>>>(cstat, cpass, h->dc, n);
+    items(h->A, 1, nullptr, rpass, cpass, 1);
+    items(h->T, 1, nullptr, cpass, rpass, 0);
+  };
+  for (int pass = 0; pass < ruiz_iterations; ++pass) {
+    items(h->A, 0, rstat, nullptr, nullptr, 1);
+    items(h->T, 0, cstat, nullptr, nullptr, 0);
+    factors_and_apply();
+  }
+  if (m > 0) k_sc_linesum<<<waves(m), kThreads, 0, h->stream>>>(h->A.rowptr, h->A.val64, m, rstat);
+  if (n > 0) k_sc_linesum<<<waves(n), kThreads, 0, h->stream>>>(h->T.rowptr, h->T.val64, n, cstat);
+  factors_and_apply();
+  const int64_t most = std::max<int64_t>(std::max<int64_t>(m, n), 1);
+  k_sc_vectors<<<lanes(most), kThreads, 0, h->stream>>>(n, m, h->dc, h->dr, h->cost, h->cl, h->cu, h->rl,
+                                                        h->ru, h->chat, h->lhat, h->uhat, h->rlhat,
+                                                        h->ruhat);
+  const int blocks = static_cast<int>(std::min<int64_t>(1024, std::max<int64_t>(1, (h->nnz + kThreads - 1) / kThreads)));
+  double* part = nullptr;
+  int rc = dalloc(h, &part, static_cast<size_t>(2 * blocks + 2));
+  if (rc) return rc;
+  k_sc_extremes<<<blocks, kThreads, 0, h->stream>>>(h->A.val64, h->nnz, part);
+  k_sc_extremes_final<<<1, kThreads, 0, h->stream>>>(part, blocks, part + 2 * blocks);
+  if (h->precision == PDHG_PRECISION_MIXED && h->nnz > 0) {
+    k_sc_narrow<<<lanes(h->nnz), kThreads, 0, h->stream>>>(h->A.val64, h->A.val32, h->nnz);
+    k_sc_narrow<<<lanes(h->nnz), kThreads, 0, h->stream>>>(h->T.val64, h->T.val32, h->nnz);
+  }
+  HIP_TRY(h, hipGetLastError());
+  double ext[2] = {0.0, 0.0};
+  HIP_TRY(h, hipMemcpyAsync(ext, part + 2 * blocks, sizeof(ext), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (extremes) {
+    extremes[0] = ext[0];
+    extremes[1] = ext[1];
+  }
+  return PDHG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -846,6 +1131,47 @@ int pdhg_dev_upload(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz, const int32_
   HIP_TRY(h, hipMemcpyAsync(h->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   h->uploaded = true;
+  return PDHG_OK;
+}
+
+int pdhg_dev_upload_unscaled(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz,
+                             const int32_t* a_rowptr, const int32_t* a_colidx, const double* a_val,
+                             const int32_t* at_rowptr, const int32_t* at_colidx,
+                             const double* at_val, const double* min_cost, const double* col_lower,
+                             const double* col_upper, const double* row_lower,
+                             const double* row_upper, double bound_norm, double cost_norm,
+                             int ruiz_iterations, double* extremes) {
+  if (!h) return PDHG_E_ARGUMENT;
+  if (ruiz_iterations < 0) return fail(h, PDHG_E_ARGUMENT, "negative Ruiz iteration count");
+  if (!min_cost && n > 0) return fail(h, PDHG_E_ARGUMENT, "null cost");
+  // The unscaled vectors stand in for the scaled ones until device_scale overwrites them.
+  int rc = pdhg_dev_upload(h, m, n, nnz, a_rowptr, a_colidx, a_val, at_rowptr, at_colidx, at_val,
+                           min_cost, col_lower, col_upper, row_lower, row_upper, nullptr, nullptr,
+                           min_cost, col_lower, col_upper, row_lower, row_upper, bound_norm,
+                           cost_norm);
+  if (rc) return rc;
+  return device_scale(h, ruiz_iterations, extremes);
+}
+
+int pdhg_dev_scaling(pdhg_dev* h, double* row, double* column, double* csc_values, double* cost_hat,
+                     double* col_lower_hat, double* col_upper_hat, double* row_lower_hat,
+                     double* row_upper_hat) {
+  int rc = check_ready(h, false);
+  if (rc) return rc;
+  HIP_TRY(h, hipSetDevice(h->device));
+  const size_t un = static_cast<size_t>(h->n), um = static_cast<size_t>(h->m);
+  const size_t nz = static_cast<size_t>(h->nnz);
+  struct Item {
+    double* dst;
+    const double* src;
+    size_t count;
+  } items[] = {{row, h->dr, um},          {column, h->dc, un},       {csc_values, h->T.val64, nz},
+               {cost_hat, h->chat, un},   {col_lower_hat, h->lhat, un}, {col_upper_hat, h->uhat, un},
+               {row_lower_hat, h->rlhat, um}, {row_upper_hat, h->ruhat, um}};
+  for (const Item& it : items)
+    if (it.dst && it.count)
+      HIP_TRY(h, hipMemcpyAsync(it.dst, it.src, sizeof(double) * it.count, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
   return PDHG_OK;
 }
 

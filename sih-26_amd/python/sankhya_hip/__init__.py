@@ -128,6 +128,8 @@ def load_library() -> C.CDLL:
     lib.pdhg_dev_promote.argtypes = [vp, C.POINTER(DevParams)]
     lib.pdhg_dev_ticks.argtypes = [vp, C.c_int64, C.POINTER(Tick), C.c_int64, C.POINTER(C.c_int64)]
     lib.pdhg_dev_finish.argtypes = [vp, _f64p, _f64p, _f64p, _f64p]
+    lib.sankhya_pdhg_csr.argtypes = [vp, C.c_int, _i32p, _i32p, _f64p]
+    lib.pdhg_dev_scaling.argtypes = [vp] + [_f64p] * 8
     lib.pdhg_dev_time_kernels.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.pdhg_dev_bytes.restype = C.c_int64
     lib.pdhg_dev_bytes.argtypes = [vp]
@@ -418,6 +420,20 @@ def solve_batch(models, **options):
     return out
 
 
+def host_csr(model: Model, threads: int = 1) -> Dict[str, np.ndarray]:
+    """The CSR of the scaled matrix the engine uploads (sankhya_pdhg_csr), built on `threads`
+    host threads."""
+    lib = load_library()
+    m, n, nnz = model.dims
+    out = dict(rowptr=np.zeros(m + 1, np.int32), colidx=np.zeros(max(nnz, 1), np.int32),
+               values=np.zeros(max(nnz, 1)))
+    rc = lib.sankhya_pdhg_csr(model._h, int(threads), out["rowptr"], out["colidx"], out["values"])
+    if rc != 0:
+        raise RuntimeError(f"sankhya_pdhg_csr failed ({rc})")
+    out["colidx"], out["values"] = out["colidx"][:nnz], out["values"][:nnz]
+    return out
+
+
 def host_scaling(model: Model, ruiz: int = 10, threads: int = 1) -> Dict[str, np.ndarray]:
     """The engine's host preconditioner (Dr, Dc, scaled CSC values, scaled vectors), on
     `threads` host threads."""
@@ -506,6 +522,21 @@ class Device:
         x, y, red, res = np.zeros(max(self.n, 1)), np.zeros(max(self.m, 1)), np.zeros(max(self.n, 1)), np.zeros(9)
         self._check(self._lib.pdhg_dev_finish(self._h, x, y, red, res), "finish")
         return x[: self.n], y[: self.m], red[: self.n], res
+
+    def scaling(self) -> Dict[str, np.ndarray]:
+        """The device's preconditioning (pdhg_dev_scaling): Dr, Dc, scaled CSC values, scaled
+        cost and bounds, under the keys host_scaling() uses."""
+        m, n, nnz = self.m, self.n, self.nnz
+        out = dict(row=np.zeros(max(m, 1)), column=np.zeros(max(n, 1)), val=np.zeros(max(nnz, 1)),
+                   cost=np.zeros(max(n, 1)), col_lower=np.zeros(max(n, 1)), col_upper=np.zeros(max(n, 1)),
+                   row_lower=np.zeros(max(m, 1)), row_upper=np.zeros(max(m, 1)))
+        self._check(self._lib.pdhg_dev_scaling(self._h, out["row"], out["column"], out["val"], out["cost"],
+                                               out["col_lower"], out["col_upper"], out["row_lower"],
+                                               out["row_upper"]), "scaling")
+        for k, ln in (("row", m), ("column", n), ("val", nnz), ("cost", n), ("col_lower", n),
+                      ("col_upper", n), ("row_lower", m), ("row_upper", m)):
+            out[k] = out[k][:ln]
+        return out
 
     def time_kernels(self, reps: int = 20):
         a, b = C.c_double(), C.c_double()

@@ -375,41 +375,30 @@ int sankhya_pdhg_scaling(const sankhya_model* h, int ruiz, int threads, double* 
   return 0;
 }
 
+int sankhya_pdhg_csr(const sankhya_model* h, int threads, int32_t* rowptr, int32_t* colidx,
+                     double* values) {
+  if (!h || !rowptr) return PDHG_E_ARGUMENT;
+  const hip_engine::Prepared p = hip_engine::prepare(h->model, threads);
+  std::memcpy(rowptr, p.a_rp.data(), sizeof(int32_t) * p.a_rp.size());
+  if (colidx && !p.a_ci.empty()) std::memcpy(colidx, p.a_ci.data(), sizeof(int32_t) * p.a_ci.size());
+  if (values && !p.a_v.empty()) std::memcpy(values, p.a_v.data(), sizeof(double) * p.a_v.size());
+  return 0;
+}
+
 int sankhya_pdhg_device_open(const sankhya_model* h, int device, int precision, pdhg_dev** out) {
   if (!h || !out) return PDHG_E_ARGUMENT;
   const Model& md = h->model;
-  const Index m = md.num_rows(), n = md.num_cols();
-  std::vector<double> cost(static_cast<std::size_t>(n));
-  for (Index j = 0; j < n; ++j) cost[j] = md.sense_multiplier() * md.col_cost[j];
-  double cs = 0.0, bs = 0.0;
-  for (const double c : cost) cs += c * c;
-  for (Index i = 0; i < m; ++i) {
-    const double b = is_finite_bound(md.row_lower[i]) ? md.row_lower[i]
-                     : (is_finite_bound(md.row_upper[i]) ? md.row_upper[i] : 0.0);
-    bs += b * b;
-  }
-  const hip_engine::Scaling s = hip_engine::build_scaling(md, cost, 10, hip_engine::host_threads());
-  // CSR of Ahat by counting transpose.
-  const auto& cp = md.matrix.column_starts();
-  const auto& ri = md.matrix.row_indices();
-  std::vector<Index> rp(static_cast<std::size_t>(m) + 1, 0), ci(s.values.size());
-  std::vector<double> cv(s.values.size());
-  for (const Index r : ri) ++rp[static_cast<std::size_t>(r) + 1];
-  for (Index i = 0; i < m; ++i) rp[i + 1] += rp[i];
-  std::vector<Index> next(rp.begin(), rp.end() - 1);
-  for (Index j = 0; j < n; ++j)
-    for (Index k = cp[j]; k < cp[j + 1]; ++k) {
-      const Index slot = next[ri[k]]++;
-      ci[slot] = j;
-      cv[slot] = s.values[k];
-    }
+  // As the engine does it: cost, norms and the CSR of A on the host, the preconditioner on
+  // the device.
+  const hip_engine::Prepared p = hip_engine::prepare_unscaled(md, hip_engine::host_threads());
   int rc = pdhg_dev_create(device, precision, out);
   if (rc) return rc;
-  rc = pdhg_dev_upload(*out, m, n, md.num_nonzeros(), rp.data(), ci.data(), cv.data(), cp.data(),
-                       ri.data(), s.values.data(), s.cost.data(), s.col_lower.data(),
-                       s.col_upper.data(), s.row_lower.data(), s.row_upper.data(), s.column.data(),
-                       s.row.data(), cost.data(), md.col_lower.data(), md.col_upper.data(),
-                       md.row_lower.data(), md.row_upper.data(), std::sqrt(bs), std::sqrt(cs));
+  rc = pdhg_dev_upload_unscaled(*out, md.num_rows(), md.num_cols(), md.num_nonzeros(),
+                                p.a_rp.data(), p.a_ci.data(), p.a_v.data(),
+                                md.matrix.column_starts().data(), md.matrix.row_indices().data(),
+                                md.matrix.values().data(), p.cost.data(), md.col_lower.data(),
+                                md.col_upper.data(), md.row_lower.data(), md.row_upper.data(),
+                                p.bound_norm, p.cost_norm, hip_engine::kRuizIterations, nullptr);
   if (rc) {
     pdhg_dev_destroy(*out);
     *out = nullptr;

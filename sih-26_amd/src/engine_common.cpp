@@ -3,6 +3,9 @@
 // only: nothing here touches the device.
 #include "engine_common.hpp"
 
+#include "parallel.hpp"
+#include "sankhya/timer.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -31,34 +34,64 @@ Residuals from_array(const double* r) {
 // (reference CsrView::build, src/la/sparse.cpp:271-322).
 void csc_to_csr(Index m, Index n, const std::vector<Index>& cp, const std::vector<Index>& ri,
                 const std::vector<double>& v, std::vector<Index>* rp, std::vector<Index>* ci,
-                std::vector<double>* cv) {
-  const std::size_t nnz = v.size();
-  rp->assign(static_cast<std::size_t>(m) + 1, 0);
-  for (std::size_t k = 0; k < nnz; ++k) ++(*rp)[static_cast<std::size_t>(ri[k]) + 1];
-  for (Index i = 0; i < m; ++i) (*rp)[i + 1] += (*rp)[i];
+                std::vector<double>* cv, int threads) {
+  const std::size_t nnz = v.size(), um = static_cast<std::size_t>(m), un = static_cast<std::size_t>(n);
+  if (nnz < 200000) threads = 1;
+  threads = std::max(1, threads);
+  const std::size_t t = static_cast<std::size_t>(threads);
+  const std::size_t chunk = std::max<std::size_t>(1, (un + t - 1) / t);
+  // Thread k takes columns [k chunk, (k + 1) chunk) and counts its entries per row; within a
+  // row, thread k's entries then follow those of threads 0..k-1, so every row keeps its
+  // columns in ascending order, as the sequential counting sort leaves them.
+  std::vector<std::vector<Index>> slot(t, std::vector<Index>(um, 0));
+  parallel_ranges(un, threads, [&](std::size_t b, std::size_t e) {
+    std::vector<Index>& cnt = slot[b / chunk];
+    for (std::size_t j = b; j < e; ++j)
+      for (Index k = cp[j]; k < cp[j + 1]; ++k) ++cnt[static_cast<std::size_t>(ri[k])];
+  });
+  rp->assign(um + 1, 0);
+  for (std::size_t i = 0; i < um; ++i) {
+    Index total = 0;
+    for (std::size_t k = 0; k < t; ++k) total += slot[k][i];
+    (*rp)[i + 1] = (*rp)[i] + total;
+  }
+  parallel_ranges(um, threads, [&](std::size_t b, std::size_t e) {
+    for (std::size_t i = b; i < e; ++i) {
+      Index at = (*rp)[i];
+      for (std::size_t k = 0; k < t; ++k) {
+        const Index c = slot[k][i];
+        slot[k][i] = at;
+        at += c;
+      }
+    }
+  });
   ci->resize(nnz);
   cv->resize(nnz);
-  std::vector<Index> next(rp->begin(), rp->end() - 1);
-  for (Index j = 0; j < n; ++j) {
-    for (Index k = cp[j]; k < cp[j + 1]; ++k) {
-      const auto slot = static_cast<std::size_t>(next[ri[k]]++);
-      (*ci)[slot] = j;
-      (*cv)[slot] = v[k];
+  Index* cip = ci->data();
+  double* cvp = cv->data();
+  parallel_ranges(un, threads, [&](std::size_t b, std::size_t e) {
+    Index* next = slot[b / chunk].data();
+    for (std::size_t j = b; j < e; ++j) {
+      for (Index k = cp[j]; k < cp[j + 1]; ++k) {
+        const auto at = static_cast<std::size_t>(next[ri[k]]++);
+        cip[at] = static_cast<Index>(j);
+        cvp[at] = v[k];
+      }
     }
-  }
+  });
 }
 
-Prepared prepare(const Model& model, int threads) {
-  Prepared p;
-  const Index rows = model.num_rows(), cols = model.num_cols();
+namespace {
+
+// Minimise-space cost and the two norms of the unscaled problem (pdhg.cpp:94-111).
+void problem_norms(const Model& model, Prepared* p) {
   const double sense = model.sense_multiplier();
-  const auto n = static_cast<std::size_t>(cols), m = static_cast<std::size_t>(rows);
-  // The unscaled problem, in minimise space (pdhg.cpp:94-111).
-  p.cost.resize(n);
-  for (std::size_t j = 0; j < n; ++j) p.cost[j] = sense * model.col_cost[j];
+  const auto n = static_cast<std::size_t>(model.num_cols()), m = static_cast<std::size_t>(model.num_rows());
+  p->cost.resize(n);
+  for (std::size_t j = 0; j < n; ++j) p->cost[j] = sense * model.col_cost[j];
   double cost_square = 0.0;
-  for (const double c : p.cost) cost_square += c * c;
-  p.cost_norm = std::sqrt(cost_square);
+  for (const double c : p->cost) cost_square += c * c;
+  p->cost_norm = std::sqrt(cost_square);
   double bound_square = 0.0;
   for (std::size_t i = 0; i < m; ++i) {
     const double b = is_finite_bound(model.row_lower[i])
@@ -66,11 +99,31 @@ Prepared prepare(const Model& model, int threads) {
                          : (is_finite_bound(model.row_upper[i]) ? model.row_upper[i] : 0.0);
     bound_square += b * b;
   }
-  p.bound_norm = std::sqrt(bound_square);
+  p->bound_norm = std::sqrt(bound_square);
+}
+
+}  // namespace
+
+Prepared prepare(const Model& model, int threads) {
+  Prepared p;
+  problem_norms(model, &p);
   // Preconditioning (pdhg.cpp:113-117) and the CSR of Ahat the device reads next to the CSC.
+  Timer timer;
   p.scaling = build_scaling(model, p.cost, kRuizIterations, threads);
-  csc_to_csr(rows, cols, model.matrix.column_starts(), model.matrix.row_indices(),
-             p.scaling.values, &p.a_rp, &p.a_ci, &p.a_v);
+  p.scaling_seconds = timer.elapsed_seconds();
+  csc_to_csr(model.num_rows(), model.num_cols(), model.matrix.column_starts(),
+             model.matrix.row_indices(), p.scaling.values, &p.a_rp, &p.a_ci, &p.a_v, threads);
+  p.csr_seconds = timer.elapsed_seconds() - p.scaling_seconds;
+  return p;
+}
+
+Prepared prepare_unscaled(const Model& model, int threads) {
+  Prepared p;
+  problem_norms(model, &p);
+  Timer timer;
+  csc_to_csr(model.num_rows(), model.num_cols(), model.matrix.column_starts(),
+             model.matrix.row_indices(), model.matrix.values(), &p.a_rp, &p.a_ci, &p.a_v, threads);
+  p.csr_seconds = timer.elapsed_seconds();
   return p;
 }
 

@@ -28,9 +28,12 @@ struct Residuals {
 };
 Residuals from_array(const double* r);
 
+// CSR of a CSC matrix, columns ascending within each row (the counting-sort transpose of
+// src/la/sparse.cpp:271-302); threads > 1 gives the same arrays (matrices under 200,000
+// nonzeros use one thread).
 void csc_to_csr(Index m, Index n, const std::vector<Index>& cp, const std::vector<Index>& ri,
                 const std::vector<double>& v, std::vector<Index>* rp, std::vector<Index>* ci,
-                std::vector<double>* cv);
+                std::vector<double>* cv, int threads = 1);
 
 // One LP as the device receives it.
 struct Prepared {
@@ -39,9 +42,13 @@ struct Prepared {
   Scaling scaling;
   std::vector<Index> a_rp, a_ci;  // CSR of Ahat
   std::vector<double> a_v;
+  double scaling_seconds = 0.0, csr_seconds = 0.0;  // host time of the two steps
 };
 // threads: host threads for the preconditioner (the batch prepares LPs in parallel with 1).
 Prepared prepare(const Model& model, int threads = 1);
+// The same without the host preconditioner: `scaling` stays empty and a_v holds A's own values,
+// for pdhg_dev_upload_unscaled, which scales on the device.
+Prepared prepare_unscaled(const Model& model, int threads = 1);
 
 std::vector<double> power_start(const Model& model, std::int64_t random_seed);
 

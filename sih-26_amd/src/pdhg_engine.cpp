@@ -72,8 +72,10 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   const double sense = model.sense_multiplier();
   const auto n = static_cast<std::size_t>(cols), m = static_cast<std::size_t>(rows);
 
-  const Prepared prep = prepare(model, host_threads());
-  const Scaling& scaling = prep.scaling;
+  const double t_checked = timer.elapsed_seconds();
+  // Host side: minimise-space cost, norms and the CSR of A; the preconditioner runs on the
+  // device (pdhg_dev_upload_unscaled).
+  const Prepared prep = prepare_unscaled(model, host_threads());
   const std::vector<double>& cost = prep.cost;
   g_last_stats.setup_seconds = timer.elapsed_seconds();
 
@@ -86,20 +88,22 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
     return device_failure(std::string(step) + " failed (code " + std::to_string(code) + "): " +
                           pdhg_dev_last_error(dev.h));
   };
-  rc = pdhg_dev_upload(dev.h, rows, cols, model.num_nonzeros(), prep.a_rp.data(), prep.a_ci.data(),
-                       prep.a_v.data(),
-                       model.matrix.column_starts().data(), model.matrix.row_indices().data(),
-                       scaling.values.data(), scaling.cost.data(), scaling.col_lower.data(),
-                       scaling.col_upper.data(), scaling.row_lower.data(), scaling.row_upper.data(),
-                       scaling.column.data(), scaling.row.data(), cost.data(), model.col_lower.data(),
-                       model.col_upper.data(), model.row_lower.data(), model.row_upper.data(),
-                       prep.bound_norm, prep.cost_norm);
+  double extremes[2] = {0.0, 0.0};
+  const double t_scale0 = timer.elapsed_seconds();
+  rc = pdhg_dev_upload_unscaled(dev.h, rows, cols, model.num_nonzeros(), prep.a_rp.data(),
+                                prep.a_ci.data(), prep.a_v.data(), model.matrix.column_starts().data(),
+                                model.matrix.row_indices().data(), model.matrix.values().data(),
+                                cost.data(), model.col_lower.data(), model.col_upper.data(),
+                                model.row_lower.data(), model.row_upper.data(), prep.bound_norm,
+                                prep.cost_norm, kRuizIterations, extremes);
   if (rc != PDHG_OK) return dev_error(rc, "upload");
 
   const std::vector<double> v0 = power_start(model, options.get_int("random_seed"));
   double spectral_norm = 1.0;
+  const double t_uploaded = timer.elapsed_seconds();
   rc = pdhg_dev_spectral_norm(dev.h, v0.data(), kPowerIterations, &spectral_norm);
   if (rc != PDHG_OK) return dev_error(rc, "spectral norm");
+  const double t_norm = timer.elapsed_seconds();
   g_last_stats.spectral_norm = spectral_norm;
 
   const double tolerance = options.get_double("pdhg_tolerance");
@@ -112,8 +116,8 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
 
   logger.info("Solving LP with MI355X restarted PDHG on {}: {} rows, {} columns, {} nonzeros", desc,
               rows, cols, model.num_nonzeros());
-  logger.info("Scaled matrix entries in [{:.3e}, {:.3e}], estimated ||A||_2 = {:.4e}", scaling.min_abs,
-              scaling.max_abs, spectral_norm);
+  logger.info("Scaled matrix entries in [{:.3e}, {:.3e}], estimated ||A||_2 = {:.4e}", extremes[0],
+              extremes[1], spectral_norm);
   logger.info("Target relative tolerance {:.1e}, restarts {}, precision {}", tolerance,
               use_restarts ? "on" : "off", mixed ? "mixed" : "fp64");
 
@@ -137,6 +141,10 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   rc = pdhg_dev_start(dev.h, &params);
   if (rc != PDHG_OK) return dev_error(rc, "start");
   const double loop_start = timer.elapsed_seconds();
+  logger.debug("Setup: validate and probe {:.4f}s, CSR {:.4f}s, device handle {:.4f}s, upload and "
+               "device preconditioning {:.4f}s, spectral norm {:.4f}s, start and graph capture "
+               "{:.4f}s", t_checked, prep.csr_seconds, t_scale0 - g_last_stats.setup_seconds,
+               t_uploaded - t_scale0, t_norm - t_uploaded, loop_start - t_norm);
 
   pdhg_state st{};
   SolveStatus stop_status = SolveStatus::kIterationLimit;
@@ -253,7 +261,12 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   report(model, res, y, reduced,
          ReportState{converged, stop_status, st.iteration, st.restarts, tolerance, stop_at_request},
          &solution);
+  const double t_report = timer.elapsed_seconds();
+  pdhg_dev_destroy(dev.h);
+  dev.h = nullptr;
   g_last_stats.total_seconds = timer.elapsed_seconds();
+  logger.debug("Teardown: finish and report {:.4f}s, device release {:.4f}s",
+               t_report - loop_start - g_last_stats.loop_seconds, g_last_stats.total_seconds - t_report);
 
   logger.info("");
   logger.info("Status: {}   objective {:.10e}   iterations {}   restarts {}   time {:.3f}s",

@@ -2,9 +2,12 @@
 // Host preconditioner (see scaling.hpp). Reference: src/la/scaling.cpp:14-159.
 #include "scaling.hpp"
 
+#include "parallel.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 namespace sankhya::hip_engine {
@@ -16,27 +19,6 @@ double scale_bound(double bound, double multiplier) {
 
 }  // namespace
 
-namespace {
-
-// f(begin, end) over [0, count) cut into `threads` contiguous ranges, one per thread.
-template <class F>
-void parallel_ranges(std::size_t count, int threads, F&& f) {
-  if (threads <= 1 || count < 2) {
-    f(std::size_t{0}, count);
-    return;
-  }
-  const std::size_t t = static_cast<std::size_t>(threads);
-  const std::size_t chunk = (count + t - 1) / t;
-  std::vector<std::thread> pool;
-  for (std::size_t k = 1; k < t; ++k) {
-    const std::size_t b = k * chunk, e = std::min(count, b + chunk);
-    if (b < e) pool.emplace_back([&f, b, e] { f(b, e); });
-  }
-  f(std::size_t{0}, std::min(count, chunk));
-  for (std::thread& th : pool) th.join();
-}
-
-}  // namespace
 
 Scaling build_scaling(const Model& model, const std::vector<double>& min_space_cost,
                       int ruiz_iterations, int threads) {
@@ -47,6 +29,7 @@ Scaling build_scaling(const Model& model, const std::vector<double>& min_space_c
   // Small models are not worth the threads.
   if (model.num_nonzeros() < 200000) threads = 1;
   threads = std::max(1, threads);
+  const bool split = threads > 1;
   Scaling s;
   s.row.assign(um, 1.0);
   s.column.assign(un, 1.0);
@@ -54,89 +37,131 @@ Scaling build_scaling(const Model& model, const std::vector<double>& min_space_c
   std::vector<double> rstat(um), cstat(un), rpass(um), cpass(un);
   // Per-thread row maxima (Ruiz): a maximum does not depend on the order it is taken in, so
   // merging the threads' maxima gives the sequential answer bit for bit.
-  std::vector<std::vector<double>> rmax(static_cast<std::size_t>(threads > 1 ? threads : 0),
+  std::vector<std::vector<double>> rmax(static_cast<std::size_t>(split ? threads : 0),
                                         std::vector<double>(um));
-  // The Pock-Chambolle sums do depend on order, so that one pass stays a single sequential CSC
-  // sweep, as in the reference.
-  const std::size_t chunk_cols =
-      std::max<std::size_t>(1, threads > 1 ? (un + threads - 1) / threads : un);
-  for (int pass = 0; pass <= ruiz_iterations; ++pass) {
-    const bool pock_chambolle = pass == ruiz_iterations;
-    if (threads == 1 || pock_chambolle) {
-      std::fill(rstat.begin(), rstat.end(), 0.0);
-      for (Index j = 0; j < n; ++j) {
-        double c = 0.0;
-        for (Index k = starts[j]; k < starts[j + 1]; ++k) {
-          const double mag = std::fabs(s.values[k]);
-          double& r = rstat[rows[k]];
-          if (pock_chambolle) {
-            r += mag;  // Pock-Chambolle: 1-norm sums (scaling.cpp:82-89)
-            c += mag;
-          } else {
-            r = std::max(r, mag);  // Ruiz: infinity norms (scaling.cpp:50-59)
-            c = std::max(c, mag);
-          }
-        }
-        cstat[j] = c;
-      }
-    } else {
-      parallel_ranges(un, threads, [&](std::size_t b, std::size_t e) {
-        std::vector<double>& rt = rmax[b / chunk_cols];
-        std::fill(rt.begin(), rt.end(), 0.0);
-        for (std::size_t j = b; j < e; ++j) {
-          double c = 0.0;
-          for (Index k = starts[j]; k < starts[j + 1]; ++k) {
-            const double mag = std::fabs(s.values[k]);
-            rt[rows[k]] = std::max(rt[rows[k]], mag);
-            c = std::max(c, mag);
-          }
-          cstat[j] = c;
-        }
-      });
-      parallel_ranges(um, threads, [&](std::size_t b, std::size_t e) {
-        for (std::size_t i = b; i < e; ++i) {
-          double r = 0.0;
-          for (const std::vector<double>& rt : rmax) r = std::max(r, rt[i]);
-          rstat[i] = r;
-        }
-      });
-    }
-    for (Index i = 0; i < m; ++i) rpass[i] = rstat[i] > 0.0 ? 1.0 / std::sqrt(rstat[i]) : 1.0;
-    for (Index j = 0; j < n; ++j) cpass[j] = cstat[j] > 0.0 ? 1.0 / std::sqrt(cstat[j]) : 1.0;
-    parallel_ranges(un, threads, [&](std::size_t b, std::size_t e) {
+  const std::size_t chunk_cols = std::max<std::size_t>(1, split ? (un + threads - 1) / threads : un);
+
+  // One sweep over the columns: first scale each entry by this round's factors,
+  // (a * r_i) * c_j (scaling.cpp:14-40), then, when the next round is a Ruiz round, take the
+  // infinity norms it needs from the scaled entries (scaling.cpp:50-59). Fusing the two gives
+  // the same values and maxima as separate passes.
+  auto sweep = [&](bool apply, bool maxima) {
+    const Index* st = starts.data();
+    const Index* rw = rows.data();
+    const double* rp = rpass.data();
+    const double* cp = cpass.data();
+    double* vals = s.values.data();
+    double* cs = cstat.data();
+    auto body = [&](auto apply_c, auto maxima_c, std::size_t b, std::size_t e, double* rt) {
+      constexpr bool kApply = decltype(apply_c)::value, kMaxima = decltype(maxima_c)::value;
+      if (kMaxima) std::fill(rt, rt + um, 0.0);
       for (std::size_t j = b; j < e; ++j) {
-        const double cj = cpass[j];
-        for (Index k = starts[j]; k < starts[j + 1]; ++k)
-          s.values[k] = s.values[k] * rpass[rows[k]] * cj;
+        const double cj = cp[j];
+        double c = 0.0;
+        for (Index k = st[j], k1 = st[j + 1]; k < k1; ++k) {
+          double v = vals[k];
+          if (kApply) {
+            v = v * rp[rw[k]] * cj;
+            vals[k] = v;
+          }
+          if (kMaxima) {
+            const double mag = std::fabs(v);
+            double& r = rt[rw[k]];
+            r = std::max(r, mag);
+            c = std::max(c, mag);
+          }
+        }
+        if (kMaxima) cs[j] = c;
+      }
+    };
+    auto run = [&](std::size_t b, std::size_t e, double* rt) {
+      using T = std::true_type;
+      using F = std::false_type;
+      if (apply && maxima) body(T{}, T{}, b, e, rt);
+      else if (apply) body(T{}, F{}, b, e, rt);
+      else body(F{}, T{}, b, e, rt);
+    };
+    if (!split) run(0, un, rstat.data());
+    else parallel_ranges(un, threads, [&](std::size_t b, std::size_t e) { run(b, e, rmax[b / chunk_cols].data()); });
+  };
+  // This round's factors r_i = 1/sqrt(stat_i), c_j = 1/sqrt(stat_j) (1 for an empty line) and
+  // the running products Dr *= r, Dc *= c; `merged` takes the row statistic from the
+  // per-thread maxima.
+  auto factors = [&](bool merged) {
+    parallel_ranges(std::max(um, un), threads, [&](std::size_t b, std::size_t e) {
+      for (std::size_t i = b; i < std::min(e, um); ++i) {
+        double r = rstat[i];
+        if (merged) {
+          r = 0.0;
+          for (const std::vector<double>& rt : rmax) r = std::max(r, rt[i]);
+        }
+        rpass[i] = r > 0.0 ? 1.0 / std::sqrt(r) : 1.0;
+        s.row[i] *= rpass[i];
+      }
+      for (std::size_t j = b; j < std::min(e, un); ++j) {
+        cpass[j] = cstat[j] > 0.0 ? 1.0 / std::sqrt(cstat[j]) : 1.0;
+        s.column[j] *= cpass[j];
       }
     });
-    for (Index i = 0; i < m; ++i) s.row[i] *= rpass[i];
-    for (Index j = 0; j < n; ++j) s.column[j] *= cpass[j];
+  };
+
+  if (ruiz_iterations > 0) sweep(false, true);
+  for (int pass = 0; pass < ruiz_iterations; ++pass) {
+    factors(split);
+    sweep(true, pass + 1 < ruiz_iterations);
   }
-  s.cost.resize(static_cast<std::size_t>(n));
-  s.col_lower.resize(static_cast<std::size_t>(n));
-  s.col_upper.resize(static_cast<std::size_t>(n));
+  // Pock-Chambolle (alpha = 1): 1-norm sums (scaling.cpp:76-100). A sum depends on the order of
+  // its terms, so this round stays one sequential sweep in CSC order, as in the reference.
+  std::fill(rstat.begin(), rstat.end(), 0.0);
   for (Index j = 0; j < n; ++j) {
-    const double dc = s.column[j];
-    s.cost[j] = min_space_cost[j] * dc;
-    s.col_lower[j] = scale_bound(model.col_lower[j], 1.0 / dc);
-    s.col_upper[j] = scale_bound(model.col_upper[j], 1.0 / dc);
-  }
-  s.row_lower.resize(static_cast<std::size_t>(m));
-  s.row_upper.resize(static_cast<std::size_t>(m));
-  for (Index i = 0; i < m; ++i) {
-    const double dr = s.row[i];
-    s.row_lower[i] = scale_bound(model.row_lower[i], dr);
-    s.row_upper[i] = scale_bound(model.row_upper[i], dr);
-  }
-  bool first = true;
-  for (const double v : s.values) {
-    const double mag = std::fabs(v);
-    s.max_abs = std::max(s.max_abs, mag);
-    if (first || mag < s.min_abs) {
-      s.min_abs = mag;
-      first = false;
+    double c = 0.0;
+    for (Index k = starts[j]; k < starts[j + 1]; ++k) {
+      const double mag = std::fabs(s.values[k]);
+      rstat[rows[k]] += mag;
+      c += mag;
     }
+    cstat[j] = c;
+  }
+  factors(false);
+  sweep(true, false);
+
+  s.cost.resize(un);
+  s.col_lower.resize(un);
+  s.col_upper.resize(un);
+  s.row_lower.resize(um);
+  s.row_upper.resize(um);
+  parallel_ranges(std::max(um, un), threads, [&](std::size_t b, std::size_t e) {
+    for (std::size_t j = b; j < std::min(e, un); ++j) {
+      const double dc = s.column[j];
+      s.cost[j] = min_space_cost[j] * dc;
+      s.col_lower[j] = scale_bound(model.col_lower[j], 1.0 / dc);
+      s.col_upper[j] = scale_bound(model.col_upper[j], 1.0 / dc);
+    }
+    for (std::size_t i = b; i < std::min(e, um); ++i) {
+      const double dr = s.row[i];
+      s.row_lower[i] = scale_bound(model.row_lower[i], dr);
+      s.row_upper[i] = scale_bound(model.row_upper[i], dr);
+    }
+  });
+  // Extremes of |Ahat| (minimum and maximum are exact in any order).
+  const std::size_t nnz = s.values.size();
+  if (nnz > 0) {
+    const std::size_t t = static_cast<std::size_t>(threads);
+    std::vector<double> lo(t, std::fabs(s.values[0])), hi(t, 0.0);
+    const std::size_t chunk = (nnz + t - 1) / t;
+    parallel_ranges(nnz, threads, [&](std::size_t b, std::size_t e) {
+      const std::size_t w = b / chunk;
+      double a = lo[w], z = hi[w];
+      for (std::size_t k = b; k < e; ++k) {
+        const double mag = std::fabs(s.values[k]);
+        a = std::min(a, mag);
+        z = std::max(z, mag);
+      }
+      lo[w] = a;
+      hi[w] = z;
+    });
+    s.min_abs = *std::min_element(lo.begin(), lo.end());
+    s.max_abs = *std::max_element(hi.begin(), hi.end());
   }
   return s;
 }

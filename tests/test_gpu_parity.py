@@ -73,6 +73,29 @@ def test_spmv_evaluate_norm_match_oracle(name, make):
     dev.close()
 
 
+@pytest.mark.parametrize("name,make", [
+    ("zipf-20000x50000", lambda: cm.generated("zipf", 20000, 50000, 10, 2)),
+    ("zipf-200k-500k", lambda: cm.generated("zipf", 200_000, 500_000, 10, 2)),
+    ("uniform-1000x3000", lambda: cm.c0(1)),
+    ("blend-500x1200", lambda: cm.generated("blend", 500, 1200, 4, 5)),
+    ("afiro", lambda: (cm.model_of(cm.netlib("afiro")), None)),
+    ("israel", lambda: (cm.model_of(cm.netlib("israel")), None)),
+    ("no_rows", lambda: (cm.model_of(cm.DEGENERATE[0][1]), None)),
+    ("rows_without_entries", lambda: (cm.model_of(cm.DEGENERATE[1][1]), None)),
+], ids=lambda v: v if isinstance(v, str) else "")
+def test_device_preconditioning_is_bit_identical_to_the_host(name, make):
+    """The single-LP engine preconditions on the device (pdhg_dev_upload_unscaled): Dr, Dc,
+    the scaled values and the scaled cost and bounds equal the host preconditioner's bit for
+    bit, and the host's is pinned to the reference's own build_scaling (tests/test_oracle.py)."""
+    mdl, _ = make()
+    host = sh.host_scaling(mdl, 10, 1)
+    dev = sh.Device(mdl)
+    got = dev.scaling()
+    dev.close()
+    for k in host:
+        assert np.array_equal(got[k], host[k]), (k, np.flatnonzero(got[k] != host[k])[:5])
+
+
 def test_full_size_spmv_zipf_200k_500k():
     mdl, d = cm.generated("zipf", 200_000, 500_000, 10, 2)
     P = orc.Problem(d)

@@ -81,6 +81,24 @@ def test_threaded_host_scaling_is_bit_identical(kind, m, n, per_col):
             assert np.array_equal(a[k], b[k]), (t, k)
 
 
+@pytest.mark.parametrize("kind,m,n,per_col", [("zipf", 20000, 50000, 10), ("uniform", 40000, 100000, 5)])
+def test_threaded_csr_is_identical_and_row_sorted(kind, m, n, per_col):
+    """The CSR of Ahat the engine uploads: the same arrays on 1, 3 and 8 host threads, equal to
+    scipy's CSR of the scaled CSC values, columns ascending in every row."""
+    import scipy.sparse as sp
+    mdl = sh.Model.generate(kind, m, n, per_col, 7)
+    d = mdl.arrays()
+    sc = sh.host_scaling(mdl, 10, 1)
+    ref = sp.csc_matrix((sc["val"], d["rowidx"], d["colptr"]), shape=(m, n)).tocsr()
+    ref.sort_indices()
+    one = sh.host_csr(mdl, 1)
+    assert np.array_equal(one["rowptr"], ref.indptr) and np.array_equal(one["colidx"], ref.indices)
+    assert np.array_equal(one["values"], ref.data)
+    for t in (3, 8):
+        many = sh.host_csr(mdl, t)
+        assert all(np.array_equal(one[k], many[k]) for k in one), t
+
+
 def test_generators_are_deterministic_and_shaped():
     m1, d1 = cm.c0(4)
     m2, d2 = cm.c0(4)
