@@ -547,7 +547,7 @@ __global__ __launch_bounds__(kThreads) void k_sc_linesum(const int32_t* ptr, con
   if (lane == 0) stat[line] = r;
 }
 
-// This round's factor 1/sqrt(stat) (1 for an empty line) and the running product d *= factor.
+// This pass's factor 1/sqrt(stat) (1 for an empty line) and the running product d *= factor.
 __global__ __launch_bounds__(kThreads) void k_sc_factor(const double* stat, double* pass, double* d,
                                                         int32_t count) {
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < count;

@@ -41,8 +41,8 @@ Scaling build_scaling(const Model& model, const std::vector<double>& min_space_c
                                         std::vector<double>(um));
   const std::size_t chunk_cols = std::max<std::size_t>(1, split ? (un + threads - 1) / threads : un);
 
-  // One sweep over the columns: first scale each entry by this round's factors,
-  // (a * r_i) * c_j (scaling.cpp:14-40), then, when the next round is a Ruiz round, take the
+  // One sweep over the columns: first scale each entry by this pass's factors,
+  // (a * r_i) * c_j (scaling.cpp:14-40), then, when the next pass is a Ruiz pass, take the
   // infinity norms it needs from the scaled entries (scaling.cpp:50-59). Fusing the two gives
   // the same values and maxima as separate passes.
   auto sweep = [&](bool apply, bool maxima) {
@@ -84,7 +84,7 @@ Scaling build_scaling(const Model& model, const std::vector<double>& min_space_c
     if (!split) run(0, un, rstat.data());
     else parallel_ranges(un, threads, [&](std::size_t b, std::size_t e) { run(b, e, rmax[b / chunk_cols].data()); });
   };
-  // This round's factors r_i = 1/sqrt(stat_i), c_j = 1/sqrt(stat_j) (1 for an empty line) and
+  // This pass's factors r_i = 1/sqrt(stat_i), c_j = 1/sqrt(stat_j) (1 for an empty line) and
   // the running products Dr *= r, Dc *= c; `merged` takes the row statistic from the
   // per-thread maxima.
   auto factors = [&](bool merged) {
@@ -111,7 +111,7 @@ Scaling build_scaling(const Model& model, const std::vector<double>& min_space_c
     sweep(true, pass + 1 < ruiz_iterations);
   }
   // Pock-Chambolle (alpha = 1): 1-norm sums (scaling.cpp:76-100). A sum depends on the order of
-  // its terms, so this round stays one sequential sweep in CSC order, as in the reference.
+  // its terms, so this pass stays one sequential sweep in CSC order, as in the reference.
   std::fill(rstat.begin(), rstat.end(), 0.0);
   for (Index j = 0; j < n; ++j) {
     double c = 0.0;
