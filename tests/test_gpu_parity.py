@@ -73,6 +73,19 @@ def test_spmv_evaluate_norm_match_oracle(name, make):
     dev.close()
 
 
+def _transposed_zipf():
+    """The transpose of the 20,000 x 50,000 Zipf matrix as an LP: its densest columns hold tens
+    of thousands of entries, so the column side of the plan has long-row segments."""
+    import scipy.sparse as sp
+    d = cm.generated("zipf", 20000, 50000, 10, 2)[1]
+    B = sp.csc_matrix((d["val"], d["rowidx"], d["colptr"]), shape=(d["m"], d["n"])).T.tocsc()
+    B.sort_indices()
+    m, n = B.shape
+    rng = np.random.default_rng(5)
+    return sh.Model.from_csc(m, n, B.indptr, B.indices, B.data, rng.uniform(-1, 1, n),
+                             np.zeros(n), np.full(n, 10.0), np.full(m, -1.0), np.full(m, 1.0))
+
+
 @pytest.mark.parametrize("name,make", [
     ("zipf-20000x50000", lambda: cm.generated("zipf", 20000, 50000, 10, 2)),
     ("zipf-200k-500k", lambda: cm.generated("zipf", 200_000, 500_000, 10, 2)),
@@ -80,6 +93,7 @@ def test_spmv_evaluate_norm_match_oracle(name, make):
     ("blend-500x1200", lambda: cm.generated("blend", 500, 1200, 4, 5)),
     ("afiro", lambda: (cm.model_of(cm.netlib("afiro")), None)),
     ("israel", lambda: (cm.model_of(cm.netlib("israel")), None)),
+    ("dense-columns", lambda: (_transposed_zipf(), None)),
     ("no_rows", lambda: (cm.model_of(cm.DEGENERATE[0][1]), None)),
     ("rows_without_entries", lambda: (cm.model_of(cm.DEGENERATE[1][1]), None)),
 ], ids=lambda v: v if isinstance(v, str) else "")
