@@ -151,6 +151,23 @@ DEGENERATE = [
 ]
 
 
+def transposed_zipf(m: int, n: int, seed: int):
+    """The transpose of the m x n Zipf matrix (10 per column) as an n x m LP: its densest columns
+    are nearly full, so the column side of the work plan has long-row segments. Bounds
+    0 <= x <= 10, -1 <= Ax <= 1 (x = 0 is feasible), cost uniform in [-1, 1] (seed 5)."""
+    import scipy.sparse as sp
+    d = generated("zipf", m, n, 10, seed)[1]
+    B = sp.csc_matrix((d["val"], d["rowidx"], d["colptr"]), shape=(d["m"], d["n"])).T.tocsc()
+    B.sort_indices()
+    rows, cols = B.shape
+    rng = np.random.default_rng(5)
+    lp = dict(m=rows, n=cols, colptr=B.indptr.astype(np.int32), rowidx=B.indices.astype(np.int32),
+              val=B.data.copy(), cost=rng.uniform(-1, 1, cols), col_lower=np.zeros(cols),
+              col_upper=np.full(cols, 10.0), row_lower=np.full(rows, -1.0),
+              row_upper=np.full(rows, 1.0), objective_offset=0.0, maximize=False)
+    return model_of(lp), lp
+
+
 def c3_sizes(count: int, seed: int = 1000):
     """Row counts of the batched workload (sankhya_hip.workloads.batched_sizes)."""
     return wl.batched_sizes(count, seed)

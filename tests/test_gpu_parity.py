@@ -41,6 +41,7 @@ def _spmv_cases():
     yield "zipf-2000x5000", lambda: cm.generated("zipf", 2000, 5000, 10, 3)
     yield "zipf-20000x50000", lambda: cm.generated("zipf", 20000, 50000, 10, 2)
     yield "blend-500x1200", lambda: cm.generated("blend", 500, 1200, 4, 5)
+    yield "dense-columns-50000x20000", lambda: cm.transposed_zipf(20000, 50000, 2)
     for nm in ("afiro", "adlittle", "share2b", "israel", "stocfor1"):
         yield nm, (lambda nm=nm: (cm.model_of(cm.netlib(nm)), cm.netlib(nm)))
 
@@ -73,19 +74,6 @@ def test_spmv_evaluate_norm_match_oracle(name, make):
     dev.close()
 
 
-def _transposed_zipf():
-    """The transpose of the 20,000 x 50,000 Zipf matrix as an LP: its densest columns hold tens
-    of thousands of entries, so the column side of the plan has long-row segments."""
-    import scipy.sparse as sp
-    d = cm.generated("zipf", 20000, 50000, 10, 2)[1]
-    B = sp.csc_matrix((d["val"], d["rowidx"], d["colptr"]), shape=(d["m"], d["n"])).T.tocsc()
-    B.sort_indices()
-    m, n = B.shape
-    rng = np.random.default_rng(5)
-    return sh.Model.from_csc(m, n, B.indptr, B.indices, B.data, rng.uniform(-1, 1, n),
-                             np.zeros(n), np.full(n, 10.0), np.full(m, -1.0), np.full(m, 1.0))
-
-
 @pytest.mark.parametrize("name,make", [
     ("zipf-20000x50000", lambda: cm.generated("zipf", 20000, 50000, 10, 2)),
     ("zipf-200k-500k", lambda: cm.generated("zipf", 200_000, 500_000, 10, 2)),
@@ -93,7 +81,7 @@ def _transposed_zipf():
     ("blend-500x1200", lambda: cm.generated("blend", 500, 1200, 4, 5)),
     ("afiro", lambda: (cm.model_of(cm.netlib("afiro")), None)),
     ("israel", lambda: (cm.model_of(cm.netlib("israel")), None)),
-    ("dense-columns", lambda: (_transposed_zipf(), None)),
+    ("dense-columns", lambda: cm.transposed_zipf(20000, 50000, 2)),
     ("no_rows", lambda: (cm.model_of(cm.DEGENERATE[0][1]), None)),
     ("rows_without_entries", lambda: (cm.model_of(cm.DEGENERATE[1][1]), None)),
 ], ids=lambda v: v if isinstance(v, str) else "")
@@ -324,6 +312,19 @@ def test_c0_agrees_with_oracle_at_1e6(seed):
     rel = _compare(d, s, r)
     assert s.stats["final_primal"] <= 1e-6 and s.stats["final_dual"] <= 1e-6 and s.stats["final_gap"] <= 1e-6
     print(f"C0 seed {seed}: gpu it={s.iterations} oracle it={r['iterations']} rel={rel:.1e}", flush=True)
+
+
+def test_dense_columns_agree_with_oracle_at_1e6():
+    """An LP whose densest columns hold nearly 5,000 entries (the transpose of the 2,000 x 5,000
+    Zipf matrix): the primal product completes those columns in its tail kernel. Same status
+    and objective as the oracle at 1e-6, and the point passes the acceptance check."""
+    mdl, d = cm.transposed_zipf(2000, 5000, 3)
+    s = sh.solve(mdl, pdhg_tolerance=1e-6, pdhg_stop_at_request=True, pdhg_polish=False,
+                 iteration_limit=200000)
+    r = orc.solve(orc.Problem(d), tolerance=1e-6, stop_at_request=True, iteration_limit=200000)
+    rel = _compare(d, s, r)
+    print(f"dense columns: gpu {s.status} {s.objective:.12g} it={s.iterations}; oracle "
+          f"{r['status']} {r['objective']:.12g} it={r['iterations']}; rel {rel:.1e}", flush=True)
 
 
 def test_engine_spectral_norm_matches_oracle():
