@@ -74,6 +74,20 @@ def test_deterministic_batches_repeat_bit_for_bit():
         assert a.stats["schedule"] == b.stats["schedule"] == -1
 
 
+def test_batch_with_dense_columns_agrees_with_oracle():
+    """A batch holding an LP with nearly full columns (the transpose of the 2,000 x 5,000 Zipf
+    matrix: long rows on the column side of its plan) next to an ordinary LP, at 1e-6: each
+    ends with the oracle's status and objective."""
+    cases = [cm.transposed_zipf(2000, 5000, 3), cm.c0(2)]
+    sols = sh.solve_batch([mdl for mdl, _ in cases], pdhg_tolerance=1e-6, pdhg_stop_at_request=True,
+                          iteration_limit=400000)
+    for (mdl, d), s in zip(cases, sols):
+        r = orc.solve(orc.Problem(d), tolerance=1e-6, stop_at_request=True, iteration_limit=400000)
+        assert s.status == r["status"], (s.status, r["status"], s.message)
+        assert abs(s.objective - r["objective"]) <= 1e-6 * max(1.0, abs(r["objective"]))
+        print(f"batch LP {mdl.dims}: {s.status} {s.objective:.12g}; oracle {r['objective']:.12g}", flush=True)
+
+
 def test_batch_of_degenerate_shapes():
     """The degenerate LPs in one batch, the LP with no rows last (its gathers then address the
     end of the concatenated arrays): each ends with its expected status and objective."""
