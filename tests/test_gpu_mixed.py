@@ -78,6 +78,21 @@ def test_zipf_1m_2m_mixed_against_fp64():
         assert max(s.stats["final_primal"], s.stats["final_dual"], s.stats["final_gap"]) <= 1e-6
 
 
+def test_dense_columns_mixed_against_fp64():
+    """Mixed precision on an LP with nearly full columns (FP32 values on the column side's
+    long-row segments and tail): polished to 1e-8 under the reference's rule, the same status as
+    the FP64 engine and the objective within 1e-7 relative."""
+    mdl, d = cm.transposed_zipf(2000, 5000, 3)
+    opts = dict(pdhg_tolerance=1e-8, iteration_limit=1000000)
+    a = sh.solve(mdl, pdhg_precision="fp64", **opts)
+    b = sh.solve(mdl, pdhg_precision="mixed", **opts)
+    print(f"dense columns: fp64 {a.status} {a.objective:.12g} it={a.iterations}; mixed {b.status} "
+          f"{b.objective:.12g} it={b.iterations} (fp32 {b.stats['fp32_iterations']})", flush=True)
+    assert a.status == b.status, (a.message, b.message)
+    assert abs(a.objective - b.objective) <= 1e-7 * max(1.0, abs(a.objective))
+    assert b.stats["fp32_iterations"] > 0
+
+
 def test_promote_continues_the_same_run():
     mdl, _ = cm.c0(2)
     norm = sh.solve(mdl, iteration_limit=1).stats["spectral_norm"]
