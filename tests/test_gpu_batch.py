@@ -88,6 +88,20 @@ def test_batch_with_dense_columns_agrees_with_oracle():
         print(f"batch LP {mdl.dims}: {s.status} {s.objective:.12g}; oracle {r['objective']:.12g}", flush=True)
 
 
+def test_batch_limits_come_back_as_statuses():
+    """Limits end a batch as statuses, never as claims (src/pdhg/pdhg.cpp:486-567): time_limit = 0
+    stops every LP with status time_limit, an iteration limit of 40 stops each LP at
+    iteration_limit within 40 iterations; the points are finite and no dual bound is claimed."""
+    models = [cm.c0(1)[0], cm.c0(2)[0]]
+    for opts, status in ((dict(time_limit=0.0), "time_limit"),
+                         (dict(iteration_limit=40, pdhg_tolerance=1e-12), "iteration_limit")):
+        sols = sh.solve_batch(models, **opts)
+        for s in sols:
+            assert s.status == status, (opts, s.status, s.message)
+            assert s.iterations <= 40 and np.all(np.isfinite(s.col_value))
+            assert math.isinf(s.dual_bound)
+
+
 def test_batch_of_degenerate_shapes():
     """The degenerate LPs in one batch, the LP with no rows last (its gathers then address the
     end of the concatenated arrays): each ends with its expected status and objective."""
