@@ -327,6 +327,33 @@ def test_dense_columns_agree_with_oracle_at_1e6():
           f"{r['status']} {r['objective']:.12g} it={r['iterations']}; rel {rel:.1e}", flush=True)
 
 
+def test_progress_stream_reports_the_objective_of_the_result(tmp_path):
+    """tests/unit/test_pdhg.cpp:560-623 without presolve (out of scope here): with progress_out
+    set, the engine writes JSONL lines whose last best_bound agrees with the reported objective
+    to 1e-4. min 3x + y + 2z + w with x fixed at 5, optimum 20."""
+    d = cm.dense_lp([[1.0, 1.0, 1.0, 0.0], [0.0, 1.0, 0.0, 1.0], [0.0, 0.0, 1.0, 1.0]],
+                    [10.0, 4.0, -math.inf], [math.inf, math.inf, 8.0], [3.0, 1.0, 2.0, 1.0])
+    d["col_lower"] = np.array([5.0, 0.0, 0.0, 0.0])
+    d["col_upper"] = np.array([5.0, math.inf, math.inf, math.inf])
+    path = tmp_path / "progress.jsonl"
+    s = sh.solve(cm.model_of(d), pdhg_tolerance=1e-10, iteration_limit=200000, pdhg_polish=False,
+                 progress_out=str(path))
+    assert s.status == "optimal", s.message
+    assert s.iterations > 1 and abs(s.objective - 20.0) <= 1e-6
+    import json
+    last, lines = None, 0
+    for line in path.read_text().splitlines():
+        if not line.strip():
+            continue
+        rec = json.loads(line)
+        assert "best_bound" in rec
+        if isinstance(rec["best_bound"], (int, float)):
+            last = rec["best_bound"]
+            lines += 1
+    assert lines > 0
+    assert abs(last - s.objective) <= 1e-4, (last, s.objective)
+
+
 def test_engine_spectral_norm_matches_oracle():
     mdl, d = cm.c0(1)
     s = sh.solve(mdl, iteration_limit=1)
