@@ -474,7 +474,15 @@ __device__ __forceinline__ double wave_abssum_ordered(const double* val, int32_t
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) {
       const int32_t count = k1 - base < 64 * kScU ? k1 - base : 64 * kScU;
-      for (int32_t e = 0; e < count; ++e) r += lds[e];
+      int32_t e = 0;
+      for (; e + 8 <= count; e += 8) {  // eight reads in flight, then eight adds in order
+        double t[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) t[q] = lds[e + q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r += t[q];
+      }
+      for (; e < count; ++e) r += lds[e];
     }
     __builtin_amdgcn_wave_barrier();
   }
