@@ -22,7 +22,6 @@
 #include "engine_stats.hpp"
 #include "sankhya/pdhg.hpp"
 #include "sankhya/timer.hpp"
-#include "scaling.hpp"
 
 namespace sankhya {
 namespace hip_engine {
