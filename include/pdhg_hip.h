@@ -77,7 +77,12 @@ int pdhg_dev_upload_unscaled(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz,
                              const double* at_val, const double* min_cost, const double* col_lower,
                              const double* col_upper, const double* row_lower,
                              const double* row_upper, double bound_norm, double cost_norm,
-                             int ruiz_iterations, double* extremes);
+                             int ruiz_iterations, int keep_unscaled, double* extremes);
+
+/* With keep_unscaled = 1 the upload above also keeps A's own values in HBM (16 bytes per
+ * nonzero more), so that this call can restore them and precondition again, as a new solve of
+ * the same LP does, without a new upload from the host. */
+int pdhg_dev_precondition(pdhg_dev* h, double* extremes);
 
 /* Test hook: the device's preconditioning results, Dr (m), Dc (n), the scaled values in CSC
  * order (nnz) and the scaled cost and bounds; any pointer may be null. */

@@ -122,6 +122,10 @@ int sankhya_pdhg_scaling(const sankhya_model* model, int ruiz_iterations, int th
                          double* col_lower_hat, double* col_upper_hat, double* row_lower_hat,
                          double* row_upper_hat);
 
+/* The power iteration's start vector (n) the engine draws for this model: std::mt19937 seeded
+ * random_seed + 1, uniform on [-1, 1) (src/la/scaling.cpp:168-171). */
+int sankhya_pdhg_power_start(const sankhya_model* model, int64_t random_seed, double* v0);
+
 /* The CSR of the scaled matrix Ahat as the host preparation builds it (host preconditioner,
  * then the counting-sort transpose of src/la/sparse.cpp:271-302, CsrView::build :315-322):
  * rowptr (m + 1), colidx and values (nnz), columns ascending within each row. threads: host

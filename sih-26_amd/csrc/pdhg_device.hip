@@ -658,6 +658,7 @@ struct DevPlan {
   int32_t ngroups = 0;
   int4 group0 = make_int4(0, 0, 0, 0);  // host copy of lr_groups[0]
   double* seg_part = nullptr;
+  double* raw64 = nullptr;  // A's own values in this layout, kept for pdhg_dev_precondition
   template <typename V>
   PlanView<V> view() const {
     PlanView<V> p;
@@ -704,6 +705,8 @@ struct pdhg_dev {
   double *part_p = nullptr, *lr_p = nullptr, *part_d = nullptr;
   double *part_er = nullptr, *part_ec = nullptr, *lr_sum = nullptr;
   Ctl* ctl = nullptr;
+  double* sc_part = nullptr;  // preconditioner scratch: per-workgroup extremes and the result
+  int ruiz = 0;               // Ruiz passes of the last device preconditioning
   Ctl* ctl_host = nullptr;
   pdhg_tick* ticks = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -987,9 +990,11 @@ int device_scale(pdhg_dev* h, int ruiz_iterations, double* extremes) {
                                                         h->ru, h->chat, h->lhat, h->uhat, h->rlhat,
                                                         h->ruhat);
   const int blocks = static_cast<int>(std::min<int64_t>(1024, std::max<int64_t>(1, (h->nnz + kThreads - 1) / kThreads)));
-  double* part = nullptr;
-  int rc = dalloc(h, &part, static_cast<size_t>(2 * blocks + 2));
-  if (rc) return rc;
+  if (!h->sc_part) {
+    const int rc = dalloc(h, &h->sc_part, static_cast<size_t>(2 * 1024 + 2));
+    if (rc) return rc;
+  }
+  double* part = h->sc_part;
   k_sc_extremes<<<blocks, kThreads, 0, h->stream>>>(h->A.val64, h->nnz, part);
   k_sc_extremes_final<<<1, kThreads, 0, h->stream>>>(part, blocks, part + 2 * blocks);
   if (h->precision == PDHG_PRECISION_MIXED && h->nnz > 0) {
@@ -1148,7 +1153,7 @@ int pdhg_dev_upload_unscaled(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz,
                              const double* at_val, const double* min_cost, const double* col_lower,
                              const double* col_upper, const double* row_lower,
                              const double* row_upper, double bound_norm, double cost_norm,
-                             int ruiz_iterations, double* extremes) {
+                             int ruiz_iterations, int keep_unscaled, double* extremes) {
   if (!h) return PDHG_E_ARGUMENT;
   if (ruiz_iterations < 0) return fail(h, PDHG_E_ARGUMENT, "negative Ruiz iteration count");
   if (!min_cost && n > 0) return fail(h, PDHG_E_ARGUMENT, "null cost");
@@ -1158,7 +1163,31 @@ int pdhg_dev_upload_unscaled(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz,
                            min_cost, col_lower, col_upper, row_lower, row_upper, bound_norm,
                            cost_norm);
   if (rc) return rc;
+  if (keep_unscaled) {
+    const size_t nz = static_cast<size_t>(nnz);
+    if ((rc = dalloc(h, &h->A.raw64, nz)) || (rc = dalloc(h, &h->T.raw64, nz))) return rc;
+    if (nz) {
+      HIP_TRY(h, hipMemcpyAsync(h->A.raw64, h->A.val64, sizeof(double) * nz, hipMemcpyDeviceToDevice, h->stream));
+      HIP_TRY(h, hipMemcpyAsync(h->T.raw64, h->T.val64, sizeof(double) * nz, hipMemcpyDeviceToDevice, h->stream));
+    }
+  }
+  h->ruiz = ruiz_iterations;
   return device_scale(h, ruiz_iterations, extremes);
+}
+
+int pdhg_dev_precondition(pdhg_dev* h, double* extremes) {
+  int rc = check_ready(h, false);
+  if (rc) return rc;
+  if (!h->A.raw64 || !h->T.raw64)
+    return fail(h, PDHG_E_STATE, "the unscaled values were not kept (keep_unscaled = 0)");
+  HIP_TRY(h, hipSetDevice(h->device));
+  const size_t nz = static_cast<size_t>(h->nnz);
+  if (nz) {
+    HIP_TRY(h, hipMemcpyAsync(h->A.val64, h->A.raw64, sizeof(double) * nz, hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(h->T.val64, h->T.raw64, sizeof(double) * nz, hipMemcpyDeviceToDevice, h->stream));
+  }
+  h->started = false;
+  return device_scale(h, h->ruiz, extremes);
 }
 
 int pdhg_dev_scaling(pdhg_dev* h, double* row, double* column, double* csc_values, double* cost_hat,

@@ -129,6 +129,8 @@ def load_library() -> C.CDLL:
     lib.pdhg_dev_ticks.argtypes = [vp, C.c_int64, C.POINTER(Tick), C.c_int64, C.POINTER(C.c_int64)]
     lib.pdhg_dev_finish.argtypes = [vp, _f64p, _f64p, _f64p, _f64p]
     lib.sankhya_pdhg_csr.argtypes = [vp, C.c_int, _i32p, _i32p, _f64p]
+    lib.sankhya_pdhg_power_start.argtypes = [vp, C.c_int64, _f64p]
+    lib.pdhg_dev_precondition.argtypes = [vp, C.POINTER(C.c_double)]
     lib.pdhg_dev_scaling.argtypes = [vp] + [_f64p] * 8
     lib.pdhg_dev_time_kernels.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.pdhg_dev_bytes.restype = C.c_int64
@@ -420,6 +422,16 @@ def solve_batch(models, **options):
     return out
 
 
+def power_start(model: Model, random_seed: int = 0) -> np.ndarray:
+    """The start vector of the engine's power iteration for this model (sankhya_pdhg_power_start)."""
+    lib = load_library()
+    n = model.dims[1]
+    v = np.zeros(max(n, 1))
+    if lib.sankhya_pdhg_power_start(model._h, int(random_seed), v) != 0:
+        raise RuntimeError("sankhya_pdhg_power_start failed")
+    return v[:n]
+
+
 def host_csr(model: Model, threads: int = 1) -> Dict[str, np.ndarray]:
     """The CSR of the scaled matrix the engine uploads (sankhya_pdhg_csr), built on `threads`
     host threads."""
@@ -522,6 +534,41 @@ class Device:
         x, y, red, res = np.zeros(max(self.n, 1)), np.zeros(max(self.m, 1)), np.zeros(max(self.n, 1)), np.zeros(9)
         self._check(self._lib.pdhg_dev_finish(self._h, x, y, red, res), "finish")
         return x[: self.n], y[: self.m], red[: self.n], res
+
+    def precondition(self):
+        """Restore the LP's own values kept in HBM and precondition again on the device
+        (pdhg_dev_precondition): the first step of a new solve of the resident LP."""
+        ext = (C.c_double * 2)()
+        self._check(self._lib.pdhg_dev_precondition(self._h, ext), "precondition")
+        return ext[0], ext[1]
+
+    def solve(self, tolerance: float = 1e-4, stop_rule: int = 0, stop_at_request: bool = False,
+              iteration_limit: int = 1000000, random_seed: int = 0, chunk: int = 0):
+        """One whole FP64 solve of the LP this handle holds in HBM, with nothing uploaded from
+        the host: precondition again from the kept values, the power iteration from the engine's
+        start vector, start, graph chunks until the device's stopping test or limit ends the run
+        (the single-LP engine's own loop, src/pdhg_engine.cpp, without time limit or callback),
+        then the final point. Returns (state, x, y, reduced, residuals); x and y are unscaled."""
+        import time as _time
+        self.precondition()
+        norm = self.spectral_norm(power_start(self.model, random_seed), 30)
+        self.start(tolerance=tolerance, iteration_limit=iteration_limit, restart=True,
+                   stop_at_request=stop_at_request, spectral_norm=norm, chunk=chunk,
+                   stop_rule=stop_rule)
+        batch = 1
+        while True:
+            t0 = _time.perf_counter()
+            st = self.run(batch)
+            dt = _time.perf_counter() - t0
+            if st.done or st.limit_hit or st.halted:
+                break
+            # Keep each wait near 20 ms, as the engine's loop does.
+            if dt < 0.01 and batch < 256:
+                batch *= 2
+            elif dt > 0.04 and batch > 1:
+                batch //= 2
+        x, y, red, res = self.finish()
+        return st, x, y, red, res
 
     def scaling(self) -> Dict[str, np.ndarray]:
         """The device's preconditioning (pdhg_dev_scaling): Dr, Dc, scaled CSC values, scaled

@@ -375,6 +375,13 @@ int sankhya_pdhg_scaling(const sankhya_model* h, int ruiz, int threads, double* 
   return 0;
 }
 
+int sankhya_pdhg_power_start(const sankhya_model* h, int64_t random_seed, double* v0) {
+  if (!h || (!v0 && h->model.num_cols() > 0)) return PDHG_E_ARGUMENT;
+  const std::vector<double> v = hip_engine::power_start(h->model, random_seed);
+  if (!v.empty()) std::memcpy(v0, v.data(), sizeof(double) * v.size());
+  return 0;
+}
+
 int sankhya_pdhg_csr(const sankhya_model* h, int threads, int32_t* rowptr, int32_t* colidx,
                      double* values) {
   if (!h || !rowptr) return PDHG_E_ARGUMENT;
@@ -398,7 +405,8 @@ int sankhya_pdhg_device_open(const sankhya_model* h, int device, int precision, 
                                 md.matrix.column_starts().data(), md.matrix.row_indices().data(),
                                 md.matrix.values().data(), p.cost.data(), md.col_lower.data(),
                                 md.col_upper.data(), md.row_lower.data(), md.row_upper.data(),
-                                p.bound_norm, p.cost_norm, hip_engine::kRuizIterations, nullptr);
+                                p.bound_norm, p.cost_norm, hip_engine::kRuizIterations,
+                                /*keep_unscaled=*/1, nullptr);
   if (rc) {
     pdhg_dev_destroy(*out);
     *out = nullptr;

@@ -94,7 +94,7 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
                                 model.matrix.row_indices().data(), model.matrix.values().data(),
                                 cost.data(), model.col_lower.data(), model.col_upper.data(),
                                 model.row_lower.data(), model.row_upper.data(), prep.bound_norm,
-                                prep.cost_norm, kRuizIterations, extremes);
+                                prep.cost_norm, kRuizIterations, /*keep_unscaled=*/0, extremes);
   if (rc != PDHG_OK) return dev_error(rc, "upload");
 
   const std::vector<double> v0 = power_start(model, options.get_int("random_seed"));

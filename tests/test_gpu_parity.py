@@ -354,6 +354,21 @@ def test_progress_stream_reports_the_objective_of_the_result(tmp_path):
     assert abs(last - s.objective) <= 1e-4, (last, s.objective)
 
 
+def test_resident_solves_repeat_the_engine_bit_for_bit():
+    """A handle that keeps the LP in HBM solves it again and again (Device.solve: preconditioning
+    from the kept values, power iteration, loop, final point, nothing uploaded): each solve ends
+    on the engine's own iteration and point, bit for bit."""
+    mdl, _ = cm.c0(1)
+    ref = sh.solve(mdl, pdhg_tolerance=1e-6, pdhg_stop_rule="relative_kkt", iteration_limit=200000)
+    assert ref.status in ("optimal", "feasible")
+    dev = sh.Device(mdl)
+    for _ in range(2):
+        st, x, y, _, _ = dev.solve(tolerance=1e-6, stop_rule=1, iteration_limit=200000)
+        assert st.done and st.iteration == ref.iterations
+        assert np.array_equal(x, ref.col_value) and np.array_equal(-y, ref.row_dual)
+    dev.close()
+
+
 def test_engine_spectral_norm_matches_oracle():
     mdl, d = cm.c0(1)
     s = sh.solve(mdl, iteration_limit=1)

@@ -99,6 +99,20 @@ def test_threaded_csr_is_identical_and_row_sorted(kind, m, n, per_col):
         assert all(np.array_equal(one[k], many[k]) for k in one), t
 
 
+def test_power_start_is_the_reference_draw():
+    """The power iteration's start vector (sankhya_pdhg_power_start) is std::mt19937 seeded
+    random_seed + 1 through libstdc++'s uniform_real_distribution(-1, 1)
+    (src/la/scaling.cpp:168-171), the oracle's restatement of both bit for bit."""
+    import ctypes as C
+    mdl, _ = cm.c0(1)
+    n = mdl.dims[1]
+    for seed in (0, 7):
+        g = orc.OrMt()
+        orc.lib().or_mt19937_seed(C.byref(g), seed + 1)
+        want = np.array([orc.lib().or_uniform_real(C.byref(g), -1.0, 1.0) for _ in range(n)])
+        assert np.array_equal(sh.power_start(mdl, seed), want)
+
+
 def test_generators_are_deterministic_and_shaped():
     m1, d1 = cm.c0(4)
     m2, d2 = cm.c0(4)
