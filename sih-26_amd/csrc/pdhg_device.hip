@@ -43,15 +43,11 @@ namespace sankhya_hip {
 
 constexpr int kDefaultChunk = 48;
 
-// Launch bounds of the two iteration kernels (a build-time knob for occupancy sweeps).
-#ifndef SANKHYA_ITEM_BOUNDS
-#define SANKHYA_ITEM_BOUNDS __launch_bounds__(kThreads)
-#endif
 
 // ---- iteration kernels ------------------------------------------------------------------
 
 template <typename V, typename X>
-__global__ SANKHYA_ITEM_BOUNDS void k_primal(PlanView<V> T, Iter<X> it, Ctl* ctl,
+__global__ __launch_bounds__(kThreads) void k_primal(PlanView<V> T, Iter<X> it, Ctl* ctl,
                                                      double* part, int force) {
   __shared__ double lds[kTile];
   __shared__ double red[4 * 2];
@@ -82,7 +78,7 @@ __global__ __launch_bounds__(kTail) void k_primal_tail(PlanView<V> T, Iter<X> it
 }
 
 template <typename V, typename X>
-__global__ SANKHYA_ITEM_BOUNDS void k_dual(PlanView<V> A, Iter<X> it, Ctl* ctl,
+__global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ctl* ctl,
                                                    double* part, int force) {
   __shared__ double lds[kTile];
   __shared__ double red[4 * 2];

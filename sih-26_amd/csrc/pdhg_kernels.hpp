@@ -30,25 +30,15 @@
 
 namespace sankhya_hip {
 
-// Work-item geometry; the defaults are the tuned values, the macros exist for sweeps.
-#ifndef SANKHYA_TILE
-#define SANKHYA_TILE 2048
-#endif
-#ifndef SANKHYA_SEG
-#define SANKHYA_SEG 4096
-#endif
-constexpr int kThreads = 256;           // 4 wave64 wavefronts
-constexpr int kTile = SANKHYA_TILE;     // nonzeros per stream item
-constexpr int kSeg = SANKHYA_SEG;       // nonzeros per long-row segment
+// Work-item geometry (measured best on the 200,000 x 500,000 Zipf LP: tiles of 1024 and 4096,
+// segments of 2048 to 8192 and 256-row items were all slower).
+constexpr int kThreads = 256;   // 4 wave64 wavefronts
+constexpr int kTile = 2048;     // nonzeros per stream item
+constexpr int kSeg = 4096;      // nonzeros per long-row segment
 constexpr int kTail = 1024;     // threads of the single-workgroup tail kernels
 constexpr int kChunk = 2048;    // long-row segment sums staged in LDS per tail round
-#ifndef SANKHYA_SEG_BATCH
-#define SANKHYA_SEG_BATCH 9  // long-row segment steps whose gathers are issued together
-#endif
-#ifndef SANKHYA_MAX_ROWS
-#define SANKHYA_MAX_ROWS 512
-#endif
-constexpr int kMaxRows = SANKHYA_MAX_ROWS;  // rows per stream item
+constexpr int kSegBatch = 9;    // long-row segment steps whose gathers are issued together
+constexpr int kMaxRows = 512;   // rows per stream item
 
 __device__ __forceinline__ bool finite_bound(double v) {
   // include/sankhya/types.hpp:89-96: absent when infinite or at/after the MPS 1e30 marker.
@@ -387,7 +377,7 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
     }
     // Gathers in batches of kBatch steps, each batch issued whole before its products are
     // added (out-of-segment lanes gather column 0 and add nothing), in the original order.
-    constexpr int kBatch = SANKHYA_SEG_BATCH;
+    constexpr int kBatch = kSegBatch;
 #pragma unroll
     for (int ub = 0; ub < U; ub += kBatch) {
       RowSum<NR> g0[kBatch], g1[kBatch];

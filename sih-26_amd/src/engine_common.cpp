@@ -4,6 +4,7 @@
 #include "engine_common.hpp"
 
 #include "parallel.hpp"
+#include "text_format.hpp"
 #include "sankhya/timer.hpp"
 
 #include <algorithm>
@@ -144,17 +145,23 @@ void report(const Model& model, const double* res9, const std::vector<double>& y
   for (std::size_t j = 0; j < n; ++j) solution->col_dual[j] = sense * reduced[j];
   for (std::size_t i = 0; i < m; ++i) solution->row_dual[i] = sense * (-y[i]);
   solution->iterations = rs.iterations;
-  const bool verifiable = rs.converged && final_r.meets_standard();
+  // Under pdhg_stop_rule=relative_kkt the device stops on the relative KKT error alone. The
+  // reference's kFeasible asserts a feasible point: the request AND absolute primal <= 1e-7
+  // (src/pdhg/pdhg_evaluate.hpp:57-60, src/pdhg/pdhg.cpp:424-436). A relative-KKT stop that
+  // misses that test claims nothing: it is reported like a stop short of the target.
+  const bool short_of_request = rs.converged && rs.relative_kkt && !final_r.meets_request(rs.tolerance);
+  const bool converged = rs.converged && !short_of_request;
+  const bool verifiable = converged && final_r.meets_standard();
   if (verifiable) {
     solution->status = SolveStatus::kOptimal;
-    solution->message = detail::format(
+    solution->message = text::format(
         "MI355X PDHG converged after {} iterations and {} restarts; absolute primal {:.3e}, dual "
         "{:.3e}, relative gap {:.3e}",
         rs.iterations, rs.restarts, final_r.absolute_primal, final_r.absolute_dual,
         final_r.gap_as_verified);
-  } else if (rs.converged) {
+  } else if (converged) {
     solution->status = SolveStatus::kFeasible;
-    solution->message = detail::format(
+    solution->message = text::format(
         "met the requested relative tolerance {:.1e} after {} iterations, but NOT the absolute "
         "standard this project verifies against (primal {:.3e} vs {:.1e}, dual {:.3e} vs {:.1e}, "
         "relative gap {:.3e} vs {:.1e}). Reported as feasible, not optimal. {}",
@@ -168,12 +175,18 @@ void report(const Model& model, const double* res9, const std::vector<double>& y
         (rs.stop_status == SolveStatus::kTimeLimit || rs.stop_status == SolveStatus::kInterrupted)
             ? rs.stop_status
             : SolveStatus::kIterationLimit;
-    solution->message = detail::format(
+    solution->message = text::format(
         "stopped at relative primal {:.3e}, dual {:.3e}, gap {:.3e} after {} iterations and {} "
         "restarts (target {:.1e})",
         final_r.primal, final_r.dual, final_r.gap, rs.iterations, rs.restarts, rs.tolerance);
+    if (short_of_request)
+      solution->message += text::format(
+          "; stopped by pdhg_stop_rule=relative_kkt (relative KKT error <= {:.1e}), but absolute "
+          "primal {:.3e} > {:.1e}, so the request is not met and no feasibility is claimed",
+          rs.tolerance, final_r.absolute_primal, tol::kPrimalFeasibility);
   }
-  solution->stopped_by = solution->status == SolveStatus::kTimeLimit     ? LimitReason::kTime
+  solution->stopped_by = short_of_request                                ? LimitReason::kNone
+                         : solution->status == SolveStatus::kTimeLimit     ? LimitReason::kTime
                          : solution->status == SolveStatus::kInterrupted ? LimitReason::kInterrupt
                          : solution->status == SolveStatus::kIterationLimit
                              ? LimitReason::kIterations

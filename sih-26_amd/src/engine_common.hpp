@@ -21,6 +21,11 @@ constexpr int kPowerIterations = 30;  // src/pdhg/pdhg.cpp:63
 struct Residuals {
   double primal = 0, dual = 0, gap = 0, primal_objective = 0, dual_objective = 0;
   double absolute_primal = 0, absolute_dual = 0, gap_as_verified = 0, complementarity = 0;
+  // src/pdhg/pdhg_evaluate.hpp:57-60.
+  [[nodiscard]] bool meets_request(double tolerance) const {
+    return primal <= tolerance && dual <= tolerance && gap <= tolerance &&
+           absolute_primal <= tol::kPrimalFeasibility;
+  }
   [[nodiscard]] bool meets_standard() const {
     return absolute_primal <= tol::kPrimalFeasibility && absolute_dual <= tol::kDualFeasibility &&
            gap_as_verified <= tol::kDualityGap && complementarity <= 1e-6;
@@ -58,6 +63,7 @@ struct ReportState {
   std::int64_t iterations, restarts;
   double tolerance;
   bool stop_at_request;
+  bool relative_kkt;  // the device stopped under pdhg_stop_rule=relative_kkt
 };
 // Duals with the reference's sign flip, status and message, dual bound, recompute_quality.
 // col_value must already hold the point; solve_seconds is left to the caller.

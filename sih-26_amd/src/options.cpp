@@ -46,8 +46,13 @@ std::vector<OptionSpec> build_registry() {
                0.0, 4294967295.0});
   s.push_back({"pdhg_restart", OptionType::Bool, true, "Restart PDHG on the KKT-error criterion.", 0, 0});
   s.push_back({"pdhg_polish", OptionType::Bool, true,
-               "Finish the point in FP64 when the bulk ran in mixed precision (an FP64 continuation "
-               "on the GPU; the interior-point finish is not part of this build).", 0, 0});
+               "Hand a PDHG point that is not optimal to the interior-point polish as a warm "
+               "start (col_value, row_dual, col_dual). solve() runs it when an interior-point "
+               "engine is installed (sankhya::install_polish_engine); the engine itself never "
+               "reads this option.", 0, 0});
+  s.push_back({"pdhg_mixed_polish", OptionType::Bool, true,
+               "Mixed precision: after the FP32 phase, continue the same run in FP64 on the GPU "
+               "to pdhg_tolerance (the FP64 polish). Off: the FP32 phase runs to the end.", 0, 0});
   s.push_back({"polish_iteration_limit", OptionType::Int, std::int64_t{50}, "Polish iterations.", 0.0, 9.0e18});
   s.push_back({"pdhg_tolerance", OptionType::Double, tol::kPdhgLoose, "PDHG relative KKT tolerance.", 0.0, 1.0});
   s.push_back({"pdhg_stop_at_request", OptionType::Bool, false,
@@ -57,7 +62,7 @@ std::vector<OptionSpec> build_registry() {
                "Bulk arithmetic: fp64, or mixed (FP32 matrix and iterates, FP64 reductions, step "
                "control and residuals).", 0, 0, {"fp64", "mixed"}});
   s.push_back({"pdhg_mixed_switch", OptionType::Double, 1e-6,
-               "Mixed precision with pdhg_polish: the FP32 phase runs until the relative KKT error "
+               "Mixed precision with pdhg_mixed_polish: the FP32 phase runs until the relative KKT error "
                "is <= max(this, pdhg_tolerance), or stalls; the same run then continues in FP64 "
                "to pdhg_tolerance.", 0.0, 1.0});
   s.push_back({"pdhg_stop_rule", OptionType::String, std::string("reference"),

@@ -201,8 +201,13 @@ std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models
     es.final_gap = res[9 * q + 2];
     es.schedule = geo[q];
     report(*models[k], res.data() + 9 * q, yl, rl2,
-           ReportState{st.done != 0, st.done ? SolveStatus::kOptimal : stop_status, st.iteration,
-                       st.restarts, tolerance, stop_at_request},
+           // An LP that had already stopped on its own iteration limit keeps that status when
+           // a time limit or an interrupt later ends the rest of the batch.
+           ReportState{st.done != 0,
+                       st.done        ? SolveStatus::kOptimal
+                       : st.limit_hit ? SolveStatus::kIterationLimit
+                                      : stop_status,
+                       st.iteration, st.restarts, tolerance, stop_at_request, params.stop_rule == 1},
            &s);
     off_n += n[q];
     off_m += m[q];

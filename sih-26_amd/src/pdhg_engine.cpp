@@ -132,7 +132,7 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   // max(pdhg_mixed_switch, tolerance); the same run then continues in FP64 under the requested
   // rule and tolerance (pdhg_dev_promote). A stalled FP32 phase hands over early.
   const pdhg_params final_params = params;
-  bool fp32_phase = mixed && options.get_bool("pdhg_polish");
+  bool fp32_phase = mixed && options.get_bool("pdhg_mixed_polish");
   if (fp32_phase) {
     params.tolerance = std::max(tolerance, options.get_double("pdhg_mixed_switch"));
     params.stop_rule = 1;
@@ -258,7 +258,8 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   g_last_stats.final_gap = final_r.gap;
   solution.solve_seconds = timer.elapsed_seconds();
   report(model, res, y, reduced,
-         ReportState{converged, stop_status, st.iteration, st.restarts, tolerance, stop_at_request},
+         ReportState{converged, stop_status, st.iteration, st.restarts, tolerance, stop_at_request,
+                     final_params.stop_rule == 1},
          &solution);
   const double t_report = timer.elapsed_seconds();
   pdhg_dev_destroy(dev.h);

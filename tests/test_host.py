@@ -217,3 +217,27 @@ def test_verifier_restatement_rejects_planted_errors():
     assert not accepted(d, bad)[0]
     bad = dict(good, row_dual=[2.0])
     assert not accepted(d, bad)[0]
+
+
+REFERENCE = "/root/reference"
+# The engine's drop-in sources: the driver behind sankhya::pdhg::solve_pdhg /
+# sankhya::gpu::solve_pdhg_gpu, its host steps, the batched driver and the threaded
+# preconditioner. SANKHYA builds them against its own headers, not against the mirror here.
+SEAM_SOURCES = ["pdhg_engine.cpp", "engine_common.cpp", "pdhg_batch_engine.cpp", "scaling.cpp"]
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE, "include", "sankhya")),
+                    reason="the reference tree is not present (it never is on the GPU box)")
+@pytest.mark.parametrize("source", SEAM_SOURCES)
+def test_engine_sources_compile_against_the_reference_headers(source):
+    """include/sankhya/pdhg.hpp:15-16 and src/gpu/pdhg_gpu.hpp:19-20 are the seam: the driver
+    must compile inside SANKHYA against SANKHYA's own Model, Solution, Options and Logger (fmt
+    from the image's torch headers, as SANKHYA pins fmt 10)."""
+    import torch
+    fmt_include = os.path.join(os.path.dirname(torch.__file__), "include")
+    src = os.path.join(ROOT, "sih-26_amd", "src")
+    cmd = ["g++", "-std=c++20", "-fsyntax-only", "-DFMT_HEADER_ONLY",
+           "-I" + os.path.join(REFERENCE, "include"), "-I" + os.path.join(REFERENCE, "src"),
+           "-I" + src, "-I" + fmt_include, os.path.join(src, source)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
