@@ -127,13 +127,13 @@ def batched_comparison(sh, count):
     t = time.perf_counter()
     sols = sh.solve_batch(models, pdhg_precision="mixed", **opts)
     tb = time.perf_counter() - t
-    solved = sum(1 for s in sols if s.status in ("optimal", "feasible"))
+    solved = sum(1 for s in sols if s.reached(1e-6))
     log(f"[bench] batched mixed: {solved}/{count} solved in {tb:.2f}s")
     sample = models[:: max(1, count // 16)]
     t = time.perf_counter()
     single = [sh.solve(m, **opts) for m in sample]
     ts = time.perf_counter() - t
-    single_ok = sum(1 for s in single if s.status in ("optimal", "feasible"))
+    single_ok = sum(1 for s in single if s.reached(1e-6))
     single_rate = single_ok / ts if ts > 0 else 0.0
     log(f"[bench] fp64 single-LP sample: {single_ok}/{len(sample)} in {ts:.2f}s")
     its = sorted(s.iterations for s in sols)
@@ -175,7 +175,7 @@ def adjoint_check(sh, count):
                           iteration_limit=2_000_000, pdhg_polish=False)
     tf = time.perf_counter() - t
     sols = ses.solutions
-    solved = [s.status in ("optimal", "feasible") for s in sols]
+    solved = [s.reached(1e-9) for s in sols]
     log(f"[bench] adjoint forward: {sum(solved)}/{count} solved in {tf:.2f}s")
     t = time.perf_counter()
     gc, gb, its, _ = ses.adjoint([d["cost"] for d in datas], None)
@@ -284,7 +284,7 @@ def main():
             f"loop {s.stats['loop_seconds']:.3f}s)")
     total = time.perf_counter() - t0
     # "Solved" = the engine stopped on the requested criterion (status optimal or feasible).
-    solved = sum(1 for s in sols if s.status in ("optimal", "feasible"))
+    solved = sum(1 for s in sols if s.reached(tol))
     value = solved / total if total > 0 else 0.0
     last = sols[-1]
     pcie = {"value": value, "unit": "LPs/s", "ms_per_solve": 1e3 * total / max(args.steps, 1),
@@ -341,7 +341,7 @@ def main():
         ts = time.perf_counter()
         sm = sh.solve(model, **dict(opts, pdhg_precision="mixed"))
         tm = time.perf_counter() - ts
-        mixed = {"value": (1.0 / tm) if sm.status in ("optimal", "feasible") else 0.0,
+        mixed = {"value": (1.0 / tm) if sm.reached(tol) else 0.0,
                  "unit": "LPs/s", "status": sm.status, "iterations": sm.iterations,
                  "fp32_iterations": sm.stats["fp32_iterations"], "seconds": tm,
                  "objective_rel_diff_vs_fp64": abs(sm.objective - last.objective) / max(1.0, abs(last.objective))}

@@ -175,9 +175,29 @@ int pdhg_dev_evaluate(pdhg_dev* h, const double* x, const double* y, double* act
  * with HIP events on the engine's stream around `reps` extra standalone launches. */
 int pdhg_dev_time_kernels(pdhg_dev* h, int reps, double* primal_ms, double* dual_ms);
 
-/* Bytes of device memory the handle holds (replaces estimate_pdhg_gpu_memory's role,
- * src/gpu/gpu_memory.cpp:18-46, with the exact figure). */
+/* Bytes of device memory the handle holds. */
 int64_t pdhg_dev_bytes(const pdhg_dev* h);
+
+/* Replaces estimate_pdhg_gpu_memory (src/gpu/gpu_memory.cpp:18-46) before anything is
+ * allocated: the device bytes a handle of this precision will hold for this LP, from the two
+ * CSR patterns the upload takes (host pointers: a_rowptr m+1 and a_colidx nnz, at_rowptr n+1
+ * and at_colidx nnz). It builds the same work plans the upload builds, so the figure is what
+ * pdhg_dev_bytes reports after pdhg_dev_upload (unscaled = 0) or pdhg_dev_upload_unscaled
+ * (unscaled = 1, keep_unscaled as passed there), plus, for a mixed handle, the FP64 iterate
+ * pairs pdhg_dev_promote adds. Host arithmetic only (no device call). Returns -1 on bad
+ * arguments. */
+int64_t pdhg_dev_estimate_bytes(int32_t m, int32_t n, int64_t nnz, const int32_t* a_rowptr,
+                                const int32_t* a_colidx, const int32_t* at_rowptr,
+                                const int32_t* at_colidx, int precision, int unscaled,
+                                int keep_unscaled);
+
+/* Replaces vram_reserve (src/gpu/gpu_memory.cpp:50-55): the headroom kept free beside the
+ * estimate, max(256 MiB, total / 10). */
+int64_t pdhg_dev_vram_reserve(int64_t total_bytes);
+
+/* Replaces device_free_memory (the dispatcher's VRAM check, src/core/solve.cpp:751-771):
+ * free and total bytes of device `device` (hipMemGetInfo). */
+int pdhg_dev_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
 
 /* Fault injection for host tests: make the k-th subsequent device call fail with
  * PDHG_E_INJECTED (k >= 1), or disable with 0. */

@@ -140,6 +140,14 @@ int sankhya_pdhg_csr(const sankhya_model* model, int threads, int32_t* rowptr, i
 int sankhya_pdhg_device_open(const sankhya_model* model, int device, int precision,
                              pdhg_dev** out);
 
+/* The engine's VRAM check on the host, as solve() makes it before allocating anything
+ * (src/core/solve.cpp:751-771): the device-memory estimate of this model for the precision
+ * (pdhg_dev_estimate_bytes) plus the reserve (pdhg_dev_vram_reserve) against free_bytes.
+ * Returns 1 when it fits, 0 when it does not, -1 on bad arguments; required and reserve (may be
+ * NULL) receive the two figures. Host arithmetic only: no device is needed. */
+int sankhya_pdhg_memory_check(const sankhya_model* model, int precision, int64_t free_bytes,
+                              int64_t total_bytes, int64_t* required, int64_t* reserve);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1073,6 +1073,60 @@ void pdhg_dev_inject_fault(pdhg_dev* h, int k) {
   if (h) h->fault_k = k > 0 ? k : 0;
 }
 
+int64_t pdhg_dev_estimate_bytes(int32_t m, int32_t n, int64_t nnz, const int32_t* a_rowptr,
+                                const int32_t* a_colidx, const int32_t* at_rowptr,
+                                const int32_t* at_colidx, int precision, int unscaled,
+                                int keep_unscaled) {
+  if (m < 0 || n < 0 || nnz < 0 || nnz > 0x7fffffffLL || !a_rowptr || !at_rowptr) return -1;
+  if (nnz > 0 && (!a_colidx || !at_colidx)) return -1;
+  if (precision != PDHG_PRECISION_FP64 && precision != PDHG_PRECISION_MIXED) return -1;
+  if (a_rowptr[m] != nnz || at_rowptr[n] != nnz) return -1;
+  // The same sizes dalloc gives every array: count + kSpare elements.
+  auto bytes = [](int64_t count, int64_t elem) { return (count + static_cast<int64_t>(kSpare)) * elem; };
+  const bool mx = precision == PDHG_PRECISION_MIXED;
+  int64_t total = 0;
+  int32_t items_a = 0, items_t = 0;
+  for (int side = 0; side < 2; ++side) {
+    const int32_t rows = side == 0 ? m : n, cols = side == 0 ? n : m;
+    const int32_t* rp = side == 0 ? a_rowptr : at_rowptr;
+    const int32_t* ci = side == 0 ? a_colidx : at_colidx;
+    std::vector<int4> logical;
+    const HostPlan hp = build_plan(rows, cols, rp, ci, &logical);
+    const int64_t ni = static_cast<int64_t>(hp.items.size()), nl = static_cast<int64_t>(hp.lr_nseg.size());
+    total += bytes(rows + 1, 4) + bytes(nnz, 4) + bytes(nnz, 8) + (mx ? bytes(nnz, 4) : 0);
+    total += bytes(ni, 16) + bytes(nl, 16) + bytes(static_cast<int64_t>(hp.groups.size()), 16);
+    total += bytes(2 * static_cast<int64_t>(hp.nseg), 8);
+    if (unscaled && keep_unscaled) total += bytes(nnz, 8);  // raw64
+    (side == 0 ? items_a : items_t) = static_cast<int32_t>(ni);
+  }
+  const int64_t xe = mx ? 4 : 8;
+  total += bytes(2 * int64_t{n}, xe) + bytes(2 * int64_t{m}, xe);     // iterate pairs
+  if (mx) total += bytes(2 * int64_t{n}, 8) + bytes(2 * int64_t{m}, 8);  // FP64 pairs (promote)
+  total += 6 * bytes(n, 8) + 6 * bytes(m, 8);                         // sums, restart, best, scratch
+  total += 7 * bytes(n, 8) + 5 * bytes(m, 8);                         // problem vectors
+  const int64_t ia = std::max(items_a, 1), it = std::max(items_t, 1);
+  total += bytes(it, 8) + bytes(1, 8) + bytes(2 * ia, 8) + bytes(1, 8) + bytes(9 * ia, 8) + bytes(9 * it, 8);
+  total += bytes(1, sizeof(Ctl)) + bytes(kTickCap, sizeof(pdhg_tick));
+  if (unscaled) total += bytes(2 * 1024 + 2, 8);  // preconditioner scratch
+  return total;
+}
+
+int64_t pdhg_dev_vram_reserve(int64_t total_bytes) {
+  const int64_t kMinReserve = int64_t{256} << 20;
+  return std::max(kMinReserve, total_bytes / 10);
+}
+
+int pdhg_dev_memory(int device, int64_t* free_bytes, int64_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return PDHG_E_ARGUMENT;
+  if (pdhg_dev_probe(device, nullptr, 0) != PDHG_OK) return PDHG_E_NO_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return PDHG_E_NO_DEVICE;
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) return PDHG_E_LAUNCH;
+  *free_bytes = static_cast<int64_t>(f);
+  *total_bytes = static_cast<int64_t>(t);
+  return PDHG_OK;
+}
+
 int pdhg_dev_upload(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz, const int32_t* a_rowptr,
                     const int32_t* a_colidx, const double* a_val, const int32_t* at_rowptr,
                     const int32_t* at_colidx, const double* at_val, const double* cost_hat,

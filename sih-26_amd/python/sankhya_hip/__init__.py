@@ -10,6 +10,7 @@ falls back to a CPU solver.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from dataclasses import dataclass, field
 from typing import Dict, Optional
@@ -256,6 +257,18 @@ class Solution:
     complementarity_violation: float
     relative_gap: float
     stats: Dict[str, float] = field(default_factory=dict)
+
+    @property
+    def relative_kkt(self) -> float:
+        """The relative KKT error max(relative primal, dual, gap) at the reported point."""
+        return max(self.stats.get("final_primal", math.inf), self.stats.get("final_dual", math.inf),
+                   self.stats.get("final_gap", math.inf))
+
+    def reached(self, tolerance: float) -> bool:
+        """True when the status claims the request (optimal, feasible) or, for a run under
+        pdhg_stop_rule=relative_kkt, when the relative KKT error is at most `tolerance`. The
+        second case claims no feasibility: its absolute primal residual may exceed 1e-7."""
+        return self.status in ("optimal", "feasible") or self.relative_kkt <= tolerance
 
 
 def _options_text(options: Dict[str, object]) -> bytes:

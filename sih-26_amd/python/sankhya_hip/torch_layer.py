@@ -58,7 +58,8 @@ class _LPSolve(torch.autograd.Function):
         bs = b.detach().cpu().numpy().astype(np.float64)
         models = [family._model(k, cs[k], bs[k]) for k in range(cs.shape[0])]
         session = BatchSession(models, **family.options)
-        bad = [s.status for s in session.solutions if s.status not in ("optimal", "feasible")]
+        tolerance = float(family.options.get("pdhg_tolerance", 1e-8))
+        bad = [s.status for s in session.solutions if not s.reached(tolerance)]
         if bad:
             session.close()
             raise RuntimeError(f"LP layer: {len(bad)} LPs not solved ({bad[0]})")

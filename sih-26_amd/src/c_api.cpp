@@ -9,6 +9,7 @@
 #include <string>
 
 #include "../../include/sankhya_pdhg.h"
+#include "c_model.hpp"
 #include "engine_common.hpp"
 #include "engine_stats.hpp"
 #include "generate.hpp"
@@ -16,9 +17,6 @@
 #include "sankhya/pdhg.hpp"
 #include "scaling.hpp"
 
-struct sankhya_model {
-  sankhya::Model model;
-};
 
 using namespace sankhya;
 
@@ -412,6 +410,25 @@ int sankhya_pdhg_device_open(const sankhya_model* h, int device, int precision, 
     *out = nullptr;
   }
   return rc;
+}
+
+int sankhya_pdhg_memory_check(const sankhya_model* h, int precision, int64_t free_bytes,
+                              int64_t total_bytes, int64_t* required, int64_t* reserve) {
+  if (!h) return -1;
+  try {
+    const Model& md = h->model;
+    const hip_engine::Prepared prep = hip_engine::prepare_unscaled(md, 1);
+    const int64_t need = pdhg_dev_estimate_bytes(
+        md.num_rows(), md.num_cols(), md.num_nonzeros(), prep.a_rp.data(), prep.a_ci.data(),
+        md.matrix.column_starts().data(), md.matrix.row_indices().data(), precision, 1, 0);
+    if (need < 0) return -1;
+    const hip_engine::MemoryCheck mc = hip_engine::check_device_memory(need, free_bytes, total_bytes);
+    if (required) *required = need;
+    if (reserve) *reserve = pdhg_dev_vram_reserve(total_bytes);
+    return mc.fits ? 1 : 0;
+  } catch (...) {
+    return -1;
+  }
 }
 
 }  // extern "C"

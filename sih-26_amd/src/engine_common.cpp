@@ -3,6 +3,8 @@
 // only: nothing here touches the device.
 #include "engine_common.hpp"
 
+#include "../../include/pdhg_hip.h"
+
 #include "parallel.hpp"
 #include "text_format.hpp"
 #include "sankhya/timer.hpp"
@@ -194,6 +196,25 @@ void report(const Model& model, const double* res9, const std::vector<double>& y
   solution->dual_bound = verifiable ? sense * final_r.dual_objective + model.objective_offset
                                     : (model.sense == ObjSense::kMaximize ? kInfinity : -kInfinity);
   solution->recompute_quality(model);
+}
+
+MemoryCheck check_device_memory(std::int64_t required, std::int64_t free_bytes,
+                                std::int64_t total_bytes) {
+  constexpr double kMiB = 1048576.0;
+  const std::int64_t reserve = pdhg_dev_vram_reserve(total_bytes);
+  MemoryCheck mc;
+  mc.required_mib = static_cast<double>(required) / kMiB;
+  mc.free_mib = static_cast<double>(free_bytes) / kMiB;
+  mc.reserve_mib = static_cast<double>(reserve) / kMiB;
+  mc.fits = required >= 0 && required + reserve <= free_bytes;
+  if (!mc.fits)
+    mc.message = required < 0
+                     ? std::string("the device memory estimate failed (malformed pattern)")
+                     : text::format("the LP needs an estimated {:.0f} MiB of device memory plus a "
+                                    "{:.0f} MiB reserve, and {:.0f} MiB are free; nothing was "
+                                    "allocated",
+                                    mc.required_mib, mc.reserve_mib, mc.free_mib);
+  return mc;
 }
 
 namespace {

@@ -6,6 +6,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "sankhya/model.hpp"
@@ -69,6 +70,17 @@ struct ReportState {
 // col_value must already hold the point; solve_seconds is left to the caller.
 void report(const Model& model, const double* res9, const std::vector<double>& y,
             const std::vector<double>& reduced, const ReportState& rs, Solution* solution);
+
+// The dispatcher's VRAM decision (src/core/solve.cpp:751-771): the estimate plus the reserve
+// (max(256 MiB, total / 10), src/gpu/gpu_memory.cpp:50-55) must fit in the free bytes. Host
+// arithmetic only.
+struct MemoryCheck {
+  bool fits;
+  double required_mib, free_mib, reserve_mib;
+  std::string message;  // set when it does not fit
+};
+MemoryCheck check_device_memory(std::int64_t required, std::int64_t free_bytes,
+                                std::int64_t total_bytes);
 
 // The device the gpu_devices option names (one GPU: the first listed, with a warning when
 // several are given).

@@ -78,8 +78,28 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   const std::vector<double>& cost = prep.cost;
   g_last_stats.setup_seconds = timer.elapsed_seconds();
 
+  // The VRAM check before anything is allocated (src/core/solve.cpp:751-771,
+  // src/gpu/gpu_memory.cpp:18-55). The reference then falls back to its CPU engine; this
+  // engine has none, so an LP that does not fit ends here with the figures in the message.
+  const int precision = mixed ? PDHG_PRECISION_MIXED : PDHG_PRECISION_FP64;
+  {
+    const std::int64_t required = pdhg_dev_estimate_bytes(
+        rows, cols, model.num_nonzeros(), prep.a_rp.data(), prep.a_ci.data(),
+        model.matrix.column_starts().data(), model.matrix.row_indices().data(), precision,
+        /*unscaled=*/1, /*keep_unscaled=*/0);
+    std::int64_t free_bytes = 0, total_bytes = 0;
+    if (pdhg_dev_memory(device, &free_bytes, &total_bytes) == PDHG_OK) {
+      // Test hook: pretend the device has this many bytes free.
+      if (const char* f = std::getenv("SANKHYA_HIP_FREE_BYTES")) free_bytes = std::atoll(f);
+      const MemoryCheck mc = check_device_memory(required, free_bytes, total_bytes);
+      logger.info("GPU PDHG memory: required {:.0f} MiB, available {:.0f} MiB, reserve {:.0f} MiB",
+                  mc.required_mib, mc.free_mib, mc.reserve_mib);
+      if (!mc.fits) return device_failure(mc.message);
+    }
+  }
+
   DevHandle dev;
-  int rc = pdhg_dev_create(device, mixed ? PDHG_PRECISION_MIXED : PDHG_PRECISION_FP64, &dev.h);
+  int rc = pdhg_dev_create(device, precision, &dev.h);
   if (rc != PDHG_OK) return device_failure("device handle creation failed (code " + std::to_string(rc) + ")");
   const int fault = static_cast<int>(std::getenv("SANKHYA_HIP_FAIL_AT") ? std::atoi(std::getenv("SANKHYA_HIP_FAIL_AT")) : 0);
   if (fault > 0) pdhg_dev_inject_fault(dev.h, fault);

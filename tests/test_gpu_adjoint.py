@@ -161,7 +161,7 @@ def test_envelope_gradient_against_simplex_duals():
     gc, gb, its, res = sess.adjoint([d["cost"] for d in datas], None)
     square = 0
     for k, (d, s) in enumerate(zip(datas, sess.solutions)):
-        if s.status not in ("optimal", "feasible"):
+        if not s.reached(1e-8):
             print(f"  LP {k}: {s.status}, not checked", flush=True)
             continue
         r, eq = cm.simplex_result(d)
@@ -192,7 +192,7 @@ def test_adjoint_against_finite_differences_of_the_exact_maps():
     print(f"CGLS iterations x {its_x[:, 0]}, y {its_y[:, 1]}; residuals {res_x[:, 0]}, {res_y[:, 1]}", flush=True)
     h = 1e-6
     for k, (d, s) in enumerate(zip(datas, sess.solutions)):
-        assert s.status in ("optimal", "feasible")
+        assert s.reached(OPTS["pdhg_tolerance"])
         x_of, rowdual_of, F = active_maps(d, s)
         b0 = d["row_upper"].copy()
         c0 = d["cost"].copy()

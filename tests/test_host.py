@@ -28,12 +28,13 @@ def declared_functions(header):
         before = text[max(0, m.start() - 40): m.start()]
         if name in ("if", "for", "while", "return", "sizeof", "defined"):
             continue
-        if re.search(r"(int|void|double|int64_t|char\s*\*|pdhg_dev\s*\*|sankhya_model\s*\*|uint32_t)\s*\**\s*$", before):
+        if re.search(r"(int|void|double|int64_t|char\s*\*|pdhg_dev\s*\*|sankhya_model\s*\*|sankhya_options\s*\*|"
+                     r"sankhya_status|sankhya_solve_status|uint32_t)\s*\**\s*$", before):
             names.add(name)
     return names
 
 
-@pytest.mark.parametrize("header", ["pdhg_hip.h", "pdhg_batch.h", "sankhya_pdhg.h"])
+@pytest.mark.parametrize("header", ["pdhg_hip.h", "pdhg_batch.h", "sankhya_pdhg.h", "sankhya_c.h"])
 def test_library_exports_every_declared_function(header):
     lib = sh.load_library()
     names = declared_functions(os.path.join(ROOT, "include", header))
