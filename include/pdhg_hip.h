@@ -170,6 +170,19 @@ int pdhg_dev_finish(pdhg_dev* h, double* col_value, double* y, double* reduced,
 int pdhg_dev_evaluate(pdhg_dev* h, const double* x, const double* y, double* activity,
                       double* reduced, double* residuals);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). The same evaluate() as
+ * pdhg_dev_evaluate, bit for bit, but every argument is a DEVICE pointer (x n, y m in; activity
+ * m, reduced n, residuals 9 out), and the partial and segment sums go to a scratch array the
+ * library holds per GPU (8 MiB, 1,048,576 doubles, kept until the process ends) instead of the
+ * handle's buffers: the call allocates nothing, host memory included. An LP whose work plans
+ * need more scratch than that (4 doubles per work item of A and of A^T plus one per long-row
+ * segment; some hundred million nonzeros) is refused with PDHG_E_ARGUMENT. Calls on one GPU
+ * share its scratch array, so calls that overlap on one GPU race; it returns after the
+ * device work has finished. */
+int pdhg_dev_evaluate_noalloc(pdhg_dev* h, const double* x, const double* y, double* activity,
+                              double* reduced, double* residuals);
+
 /* Timing hook: average device duration in milliseconds of the last `pdhg_dev_run` call's
  * kernels of one kind (0 = primal kernel over Ahat^T, 1 = dual kernel over Ahat), measured
  * with HIP events on the engine's stream around `reps` extra standalone launches. */

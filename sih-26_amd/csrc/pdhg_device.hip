@@ -29,8 +29,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -286,7 +288,8 @@ __global__ __launch_bounds__(kTail) void k_single_decide(PlanView<V> A, PlanView
                                                          Ctl* ctl, const double* bx,
                                                          const double* by, double* act_out,
                                                          double* red_out, const double* pr,
-                                                         int npr, const double* pc, int npc) {
+                                                         int npr, const double* pc, int npc,
+                                                         double* res9) {
   __shared__ double lds[kChunk];
   __shared__ double red[16 * 4];
   SingleRowsEpi<X> repi{it, by, act_out};
@@ -312,15 +315,15 @@ __global__ __launch_bounds__(kTail) void k_single_decide(PlanView<V> A, PlanView
   // rows: 0 pv 1 sup 2 dv 3 cm ; cols: 0 dv 1 bc 2 po 3 cm
   const Res r = make_res(vr[0], vc[0] + vr[2], smax(vr[3], vc[3]), vc[2], vc[1] - vr[1],
                          ctl->bound_norm, ctl->cost_norm);
-  ctl->final_res[0] = r.primal;
-  ctl->final_res[1] = r.dual;
-  ctl->final_res[2] = r.gap;
-  ctl->final_res[3] = r.pobj;
-  ctl->final_res[4] = r.dobj;
-  ctl->final_res[5] = r.abs_p;
-  ctl->final_res[6] = r.abs_d;
-  ctl->final_res[7] = r.gap_v;
-  ctl->final_res[8] = r.compl_;
+  res9[0] = r.primal;
+  res9[1] = r.dual;
+  res9[2] = r.gap;
+  res9[3] = r.pobj;
+  res9[4] = r.dobj;
+  res9[5] = r.abs_p;
+  res9[6] = r.abs_d;
+  res9[7] = r.gap_v;
+  res9[8] = r.compl_;
 }
 
 // Plain products for the test hook and the power iteration. With `scale` set, the input is
@@ -677,6 +680,31 @@ struct DevPlan {
 
 }  // namespace sankhya_hip
 
+// ---- scratch of the entry point that allocates nothing -----------------------------------
+// pdhg_dev_evaluate_noalloc keeps its per-workgroup partial sums and long-row segment sums in
+// this array: one copy per GPU, held until the process ends, shared by every call on that GPU.
+// Host code never uses the name (a host-side stand-in): it uses the device address that
+// hipGetSymbolAddress returns with the device current, recorded per device below.
+constexpr size_t kScratchDoubles = size_t{1} << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch[kMaxDevices] = {};
+
+// With `device` current: its copy of the scratch array (null when the lookup fails).
+double* scratch_for(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch[device] = static_cast<double*>(p);
+  }
+  return g_scratch[device];
+}
+}  // namespace
+
 using namespace sankhya_hip;
 
 struct pdhg_dev {
@@ -891,18 +919,30 @@ void launch_chunk(pdhg_dev* h) {
   k_clear<<<1, 64, 0, h->stream>>>(h->ctl);
 }
 
+// evaluate() of the point (bx, by) in original units: activity into act (may be null), reduced
+// costs into red, the nine residuals into res9. The partial sums go to part_r (4 per item of
+// Ahat) and part_c (4 per item of Ahat^T), the long-row segment sums to the views' seg_part.
+template <typename V, typename X>
+void launch_single_into(pdhg_dev* h, const PlanView<V>& A, const PlanView<V>& T, const double* bx,
+                        const double* by, double* act, double* red, double* part_r, double* part_c,
+                        double* res9) {
+  const Iter<X> it = iter_of<X>(h);
+  if (A.nitems > 0)
+    k_single_rows<V, X><<<A.nitems, kThreads, 0, h->stream>>>(A, it, bx, by, act, part_r);
+  if (T.nitems > 0)
+    k_single_cols<V, X><<<T.nitems, kThreads, 0, h->stream>>>(T, it, bx, by, red, part_c);
+  k_single_decide<V, X><<<1, kTail, 0, h->stream>>>(A, T, it, h->ctl, bx, by, act, red, part_r,
+                                                    A.nitems, part_c, T.nitems, res9);
+}
+
+double* final_res_of(const pdhg_dev* h) {
+  return reinterpret_cast<double*>(reinterpret_cast<char*>(h->ctl) + offsetof(Ctl, final_res));
+}
+
 template <typename V, typename X>
 void launch_single(pdhg_dev* h, const double* bx, const double* by, double* act, double* red) {
-  const Iter<X> it = iter_of<X>(h);
-  if (h->A.nitems > 0)
-    k_single_rows<V, X><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<V>(), it, bx, by, act,
-                                                                 h->part_er);
-  if (h->T.nitems > 0)
-    k_single_cols<V, X><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<V>(), it, bx, by, red,
-                                                                 h->part_ec);
-  k_single_decide<V, X><<<1, kTail, 0, h->stream>>>(h->A.view<V>(), h->T.view<V>(), it, h->ctl, bx,
-                                                    by, act, red, h->part_er, h->A.nitems,
-                                                    h->part_ec, h->T.nitems);
+  launch_single_into<V, X>(h, h->A.view<V>(), h->T.view<V>(), bx, by, act, red, h->part_er,
+                           h->part_ec, final_res_of(h));
 }
 
 // y = P x through the same work items; long rows completed by the tail kernel. With `scale`
@@ -1040,6 +1080,8 @@ int pdhg_dev_create(int device, int precision, pdhg_dev** out) {
   h->device = device;
   h->precision = precision;
   h->arith = precision;
+  h->err.reserve(512);  // error texts of the no-allocation entry point fit without allocating
+  (void)scratch_for(device);  // this device's scratch address, looked up once at set-up
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return PDHG_E_LAUNCH;
@@ -1478,6 +1520,41 @@ int pdhg_dev_evaluate(pdhg_dev* h, const double* x, const double* y, double* act
   HIP_TRY(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   if (residuals) std::memcpy(residuals, h->ctl_host->final_res, sizeof(double) * 9);
+  return PDHG_OK;
+}
+
+int pdhg_dev_evaluate_noalloc(pdhg_dev* h, const double* x, const double* y, double* activity,
+                              double* reduced, double* residuals) {
+  // No allocation anywhere on this path, error texts included (h->err has room reserved).
+  if (!h) return PDHG_E_ARGUMENT;
+  auto refuse = [h](int code, const char* what) {
+    h->err.assign(what);
+    return code;
+  };
+  if (!h->uploaded) return refuse(PDHG_E_STATE, "pdhg_dev_evaluate_noalloc: no problem uploaded");
+  if (injected(h)) return PDHG_E_INJECTED;
+  if ((h->n > 0 && (!x || !reduced)) || (h->m > 0 && (!y || !activity)) || !residuals)
+    return refuse(PDHG_E_ARGUMENT, "pdhg_dev_evaluate_noalloc: null device pointer");
+  if (hipSetDevice(h->device) != hipSuccess)
+    return refuse(PDHG_E_LAUNCH, "pdhg_dev_evaluate_noalloc: hipSetDevice failed");
+  double* base = scratch_for(h->device);
+  if (!base) return refuse(PDHG_E_LAUNCH, "pdhg_dev_evaluate_noalloc: hipGetSymbolAddress of the scratch array failed");
+  const size_t na = static_cast<size_t>(h->A.nitems), nt = static_cast<size_t>(h->T.nitems);
+  const size_t need = 4 * (na + nt) + static_cast<size_t>(h->A.nseg) + static_cast<size_t>(h->T.nseg);
+  if (need > kScratchDoubles)
+    return refuse(PDHG_E_ARGUMENT,
+                  "pdhg_dev_evaluate_noalloc: the LP's work plans need more than the 1,048,576 "
+                  "doubles (8 MiB) of scratch the library holds per GPU; use pdhg_dev_evaluate");
+  // Layout: 4 partials per item of Ahat, 4 per item of Ahat^T, then the segment sums of each.
+  PlanView<double> A = h->A.view<double>(), T = h->T.view<double>();
+  A.seg_part = base + 4 * (na + nt);
+  T.seg_part = A.seg_part + h->A.nseg;
+  launch_single_into<double, double>(h, A, T, x, y, activity, reduced, base, base + 4 * na,
+                                     residuals);
+  if (hipGetLastError() != hipSuccess)
+    return refuse(PDHG_E_LAUNCH, "pdhg_dev_evaluate_noalloc: kernel launch failed");
+  if (hipStreamSynchronize(h->stream) != hipSuccess)
+    return refuse(PDHG_E_LAUNCH, "pdhg_dev_evaluate_noalloc: the evaluation failed on the device");
   return PDHG_OK;
 }
 
