@@ -140,6 +140,15 @@ int sankhya_pdhg_csr(const sankhya_model* model, int threads, int32_t* rowptr, i
 int sankhya_pdhg_device_open(const sankhya_model* model, int device, int precision,
                              pdhg_dev** out);
 
+/* Solution::recompute_quality of the engine (the quality fields solve() reports; reference
+ * src/core/model.cpp:247-441) for the point (col_value n, row_dual m, col_dual n) with the given
+ * dual bound. row_activity (m) out; out[9] = objective, primal_infeasibility,
+ * primal_infeasibility_scaled, dual_infeasibility, dual_infeasibility_scaled,
+ * complementarity_violation, integrality_violation, absolute_gap, relative_gap. Host only. */
+int sankhya_pdhg_quality(const sankhya_model* model, const double* col_value, const double* row_dual,
+                         const double* col_dual, double dual_bound, double* row_activity,
+                         double* out);
+
 /* The engine's VRAM check on the host, as solve() makes it before allocating anything
  * (src/core/solve.cpp:751-771): the device-memory estimate of this model for the precision
  * (pdhg_dev_estimate_bytes) plus the reserve (pdhg_dev_vram_reserve) against free_bytes.

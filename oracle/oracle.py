@@ -87,6 +87,8 @@ def lib():
         L.or_solve_pdhg.restype = C.c_int
         L.or_solve_pdhg.argtypes = [C.POINTER(OrModel), C.POINTER(OrOptions), _f64p, _f64p, _f64p,
                                     C.POINTER(OrResult), C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]
+        L.or_recompute_quality.argtypes = [C.POINTER(OrModel), _f64p, _f64p, _f64p, C.c_double,
+                                           _f64p, C.POINTER(OrResult)]
         L.or_mt19937_seed.argtypes = [C.POINTER(OrMt), C.c_uint32]
         L.or_mt19937_next.restype = C.c_uint32
         L.or_mt19937_next.argtypes = [C.POINTER(OrMt)]
@@ -107,6 +109,10 @@ def ref() -> Optional[C.CDLL]:
         R.ref_spectral_norm.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, C.c_int, C.c_uint32]
         R.ref_evaluate.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p] + [_f64p] * 6 + \
             [C.c_double, C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p]
+        R.ref_recompute_quality.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p] + [_f64p] * 5 + \
+            [C.c_double, C.c_int, _f64p, _f64p, _f64p, C.c_double, _f64p, _f64p]
+        R.ref_validate.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p] + [_f64p] * 5 + \
+            [C.c_char_p, C.c_int]
         _ref = R
     return _ref
 
@@ -205,3 +211,23 @@ def solve(P: Problem, tolerance=1e-4, iteration_limit=-1, restart=True, stop_at_
     out["ticks"] = [ticks[k] for k in range(min(nt.value, max_ticks))]
     out["num_ticks"] = nt.value
     return out
+
+
+QUALITY_FIELDS = ("objective", "primal_infeasibility", "primal_infeasibility_scaled",
+                  "dual_infeasibility", "dual_infeasibility_scaled", "complementarity_violation",
+                  "integrality_violation", "absolute_gap", "relative_gap")
+
+
+def recompute_quality(P: Problem, x, row_dual, col_dual, dual_bound):
+    """or_recompute_quality: (row activity, the nine quality fields in QUALITY_FIELDS order; an LP
+    has no integrality, so that field is 0)."""
+    act = np.zeros(max(P.m, 1))
+    q = OrResult()
+    lib().or_recompute_quality(C.byref(P.c_model()), np.ascontiguousarray(x, np.float64),
+                               np.ascontiguousarray(row_dual, np.float64),
+                               np.ascontiguousarray(col_dual, np.float64), float(dual_bound), act,
+                               C.byref(q))
+    vals = np.array([q.objective, q.primal_infeasibility, q.primal_infeasibility_scaled,
+                     q.dual_infeasibility, q.dual_infeasibility_scaled, q.complementarity_violation,
+                     0.0, q.absolute_gap, q.relative_gap])
+    return act[: P.m], vals

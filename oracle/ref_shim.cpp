@@ -1,10 +1,12 @@
-// ref_shim.cpp - a C-ABI caller of the reference's own scaling, power iteration, SpMV and
-// evaluate() functions, compiled together with those reference sources where they lie
+// ref_shim.cpp - a C-ABI caller of the reference's own scaling, power iteration, SpMV,
+// evaluate(), Solution::recompute_quality and Model::validate functions, compiled together with those reference sources where they lie
 // (oracle/Makefile, target `ref`). It exists only to check the C restatement in
 // pdhg_oracle.c against the reference bit for bit; nothing here re-implements the
 // reference. Test infrastructure only.
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "la/scaling.hpp"
@@ -87,6 +89,47 @@ void ref_evaluate(int32_t m, int32_t n, const int32_t* colptr, const int32_t* ro
                           r.primal_objective, r.dual_objective, r.absolute_primal,
                           r.absolute_dual,   r.gap_as_verified, r.complementarity};
   std::memcpy(out, vals, sizeof(vals));
+}
+
+// Solution::recompute_quality (src/core/model.cpp:247-441) of the point (x, row_dual, col_dual)
+// with the given dual bound on the model in its own sense. activity (m) out; out[9] =
+// objective, primal_infeasibility, primal_infeasibility_scaled, dual_infeasibility,
+// dual_infeasibility_scaled, complementarity_violation, integrality_violation, absolute_gap,
+// relative_gap.
+void ref_recompute_quality(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
+                           const double* val, const double* cost, const double* lo,
+                           const double* hi, const double* rlo, const double* rhi, double offset,
+                           int maximize, const double* x, const double* row_dual,
+                           const double* col_dual, double dual_bound, double* activity,
+                           double* out) {
+  Model model = make_model(m, n, colptr, rowidx, val, cost, lo, hi, rlo, rhi);
+  model.objective_offset = offset;
+  model.sense = maximize ? ObjSense::kMaximize : ObjSense::kMinimize;
+  Solution s;
+  s.col_value.assign(x, x + n);
+  s.row_dual.assign(row_dual, row_dual + m);
+  s.col_dual.assign(col_dual, col_dual + n);
+  s.dual_bound = dual_bound;
+  s.recompute_quality(model);
+  if (m > 0) std::memcpy(activity, s.row_activity.data(), sizeof(double) * m);
+  const double vals[9] = {s.objective,          s.primal_infeasibility,
+                          s.primal_infeasibility_scaled, s.dual_infeasibility,
+                          s.dual_infeasibility_scaled,   s.complementarity_violation,
+                          s.integrality_violation,       s.absolute_gap,
+                          s.relative_gap};
+  std::memcpy(out, vals, sizeof(vals));
+}
+
+// Model::validate (src/core/model.cpp:127-237) of the model as given (entries added one by one
+// and finalised without a drop tolerance). The message goes to buf (empty when well formed).
+void ref_validate(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
+                  const double* val, const double* cost, const double* lo, const double* hi,
+                  const double* rlo, const double* rhi, char* buf, int len) {
+  const Model model = make_model(m, n, colptr, rowidx, val, cost, lo, hi, rlo, rhi);
+  const std::string text = model.validate();
+  const std::size_t k = std::min<std::size_t>(static_cast<std::size_t>(len - 1), text.size());
+  std::memcpy(buf, text.data(), k);
+  buf[k] = '\0';
 }
 
 }  // extern "C"

@@ -130,6 +130,7 @@ def load_library() -> C.CDLL:
     lib.pdhg_dev_ticks.argtypes = [vp, C.c_int64, C.POINTER(Tick), C.c_int64, C.POINTER(C.c_int64)]
     lib.pdhg_dev_finish.argtypes = [vp, _f64p, _f64p, _f64p, _f64p]
     lib.sankhya_pdhg_csr.argtypes = [vp, C.c_int, _i32p, _i32p, _f64p]
+    lib.sankhya_pdhg_quality.argtypes = [vp, _f64p, _f64p, _f64p, C.c_double, _f64p, _f64p]
     lib.sankhya_pdhg_power_start.argtypes = [vp, C.c_int64, _f64p]
     lib.pdhg_dev_precondition.argtypes = [vp, C.POINTER(C.c_double)]
     lib.pdhg_dev_scaling.argtypes = [vp] + [_f64p] * 8
@@ -457,6 +458,22 @@ def host_csr(model: Model, threads: int = 1) -> Dict[str, np.ndarray]:
         raise RuntimeError(f"sankhya_pdhg_csr failed ({rc})")
     out["colidx"], out["values"] = out["colidx"][:nnz], out["values"][:nnz]
     return out
+
+
+def quality(model: Model, col_value, row_dual, col_dual, dual_bound: float):
+    """The engine's Solution::recompute_quality on the host (sankhya_pdhg_quality): the row
+    activity and [objective, primal_infeasibility, primal_infeasibility_scaled,
+    dual_infeasibility, dual_infeasibility_scaled, complementarity_violation,
+    integrality_violation, absolute_gap, relative_gap]."""
+    lib = load_library()
+    m, n, _ = model.dims
+    act, out = np.zeros(max(m, 1)), np.zeros(9)
+    f = lambda v, k: np.ascontiguousarray(v if k else np.zeros(1), np.float64)
+    rc = lib.sankhya_pdhg_quality(model._h, f(col_value, n), f(row_dual, m), f(col_dual, n),
+                                  float(dual_bound), act, out)
+    if rc != 0:
+        raise RuntimeError("sankhya_pdhg_quality failed")
+    return act[:m], out
 
 
 def host_scaling(model: Model, ruiz: int = 10, threads: int = 1) -> Dict[str, np.ndarray]:

@@ -412,6 +412,29 @@ int sankhya_pdhg_device_open(const sankhya_model* h, int device, int precision, 
   return rc;
 }
 
+int sankhya_pdhg_quality(const sankhya_model* h, const double* col_value, const double* row_dual,
+                         const double* col_dual, double dual_bound, double* row_activity,
+                         double* out) {
+  if (!h || !out) return -1;
+  const Model& md = h->model;
+  const auto n = static_cast<std::size_t>(md.num_cols()), m = static_cast<std::size_t>(md.num_rows());
+  if ((n && (!col_value || !col_dual)) || (m && (!row_dual || !row_activity))) return -1;
+  Solution s;
+  s.col_value.assign(col_value, col_value + n);
+  s.row_dual.assign(row_dual, row_dual + m);
+  s.col_dual.assign(col_dual, col_dual + n);
+  s.dual_bound = dual_bound;
+  s.recompute_quality(md);
+  if (m) std::memcpy(row_activity, s.row_activity.data(), sizeof(double) * m);
+  const double vals[9] = {s.objective,          s.primal_infeasibility,
+                          s.primal_infeasibility_scaled, s.dual_infeasibility,
+                          s.dual_infeasibility_scaled,   s.complementarity_violation,
+                          s.integrality_violation,       s.absolute_gap,
+                          s.relative_gap};
+  std::memcpy(out, vals, sizeof(vals));
+  return 0;
+}
+
 int sankhya_pdhg_memory_check(const sankhya_model* h, int precision, int64_t free_bytes,
                               int64_t total_bytes, int64_t* required, int64_t* reserve) {
   if (!h) return -1;

@@ -225,12 +225,14 @@ def _solve(L, h, **opts):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", [
-    ("MinimisesASum", [[1.0, 1.0]], [1.0], [math.inf], [1.0, 1.0], None, False, 0.0, 2.0),
-    ("HonoursAnEqualityRow", [[1.0, 1.0]], [4.0], [4.0], [1.0, 1.0], None, False, 0.0, 4.0),
-    ("HonoursARangeRow", [[1.0, 1.0]], [2.0], [5.0], [1.0, 1.0], None, False, 0.0, 2.0),
+    # tests/unit/test_pdhg.cpp:95-146, built with make_lp's defaults (x >= 0, no upper bound).
+    ("MinimisesASum", [[1.0, 1.0]], [2.0], [math.inf], [1.0, 1.0], None, False, 0.0, 2.0),
+    ("HonoursAnEqualityRow", [[1.0, 1.0]], [4.0], [4.0], [1.0, 2.0], None, False, 0.0, 4.0),
+    ("HonoursARangeRow", [[1.0]], [2.0], [6.0], [1.0], None, False, 0.0, 2.0),
+    ("HonoursAColumnUpperBound", [[1.0]], [-math.inf], [100.0], [-1.0], [7.0], False, 0.0, -7.0),
     ("ReportsTheObjectiveInTheOriginalSense", [[1.0, 0.0], [0.0, 2.0], [3.0, 2.0]],
      [-math.inf] * 3, [4.0, 12.0, 18.0], [3.0, 5.0], None, True, 0.0, 36.0),
-    ("CarriesTheObjectiveOffset", [[1.0, 1.0]], [1.0], [math.inf], [1.0, 1.0], None, False, 11.0, 13.0),
+    ("CarriesTheObjectiveOffset", [[1.0]], [3.0], [math.inf], [1.0], None, False, 10.0, 13.0),
 ], ids=lambda c: c[0])
 def test_c_api_solves_the_reference_known_answers(case):
     cm.gpu_or_skip()
@@ -241,7 +243,7 @@ def test_c_api_solves_the_reference_known_answers(case):
     status = STATUS[L.sankhya_solution_status(sol)]
     obj = L.sankhya_solution_objective(sol)
     print(f"{name}: {status} {obj:.12g} ({L.sankhya_solution_algorithm(sol).decode()})", flush=True)
-    assert status == "optimal" and abs(obj - want) <= 1e-6 * max(1.0, abs(want))
+    assert status == "optimal" and abs(obj - want) <= 1e-5 * max(1.0, abs(want))
     n, m = len(cost), len(rows)
     x = np.zeros(n)
     assert L.sankhya_solution_col_values(sol, x.ctypes.data, n) == 0

@@ -27,6 +27,7 @@ import pytest
 
 import common as cm
 import oracle as orc
+import sankhya_hip as sh
 
 GOLDEN = os.path.join(cm.HERE, "golden", "ref_vectors.npz")
 CASES = ["afiro", "blend", "uniform_100x300_s5", "zipf_300x800_s4"]
@@ -252,3 +253,116 @@ def test_netlib_reader_matches_published_dimensions():
         assert d["m"] + 1 == info["published_rows"], name
         assert d["n"] == info["published_cols"], name
         assert int(d["colptr"][-1]) + d["cost_nonzeros"] == info["published_nonzeros"], name
+
+
+# ---- Solution::recompute_quality and Model::validate (src/core/model.cpp) ------------------
+
+def _quality_cases():
+    """(name, lp, point) with points that exercise every term: primal violations of both signs
+    on bounded and unbounded sides, duals of both signs on equality, one-sided and range rows,
+    reduced costs that disagree with c - A^T y, fixed columns, maximise sense and an offset."""
+    base = [("C0-s1", cm.c0(1)[1]), ("zipf-2000x5000", cm.generated("zipf", 2000, 5000, 10, 3)[1])]
+    for nm in ("afiro", "sc50a", "blend", "adlittle", "share2b", "israel", "stocfor1"):
+        base.append((nm, cm.netlib(nm)))
+    for nm, lp, _, _ in cm.DEGENERATE:
+        base.append((nm, lp))
+    out = []
+    for k, (nm, d) in enumerate(base):
+        rng = np.random.default_rng(100 + k)
+        for variant in ("min", "max+offset"):
+            lp = dict(d)
+            if variant != "min":
+                lp = dict(d, maximize=True, objective_offset=3.25)
+            P = orc.Problem(lp)
+            lo = np.where(np.abs(P.col_lower) < 1e30, P.col_lower, -5.0)
+            hi = np.where(np.abs(P.col_upper) < 1e30, P.col_upper, 5.0)
+            x = rng.uniform(lo - 0.1, hi + 0.1) if P.n else np.zeros(0)
+            if P.n:
+                x[rng.random(P.n) < 0.3] = 0.0  # zero columns are skipped by the row scale
+            yd = rng.standard_normal(P.m)
+            yd[rng.random(P.m) < 0.2] = 0.0
+            aty = orc.csc_transpose_multiply(P, yd) if P.m and P.n else np.zeros(P.n)
+            dj = P.cost - aty
+            if P.n:
+                dj[rng.random(P.n) < 0.3] += rng.standard_normal() * 1e-3
+            out.append((f"{nm}-{variant}", lp, x, yd, dj, float(rng.normal() * 10)))
+    return out
+
+
+QUALITY_CASES = _quality_cases()
+
+
+@pytest.mark.skipif(not os.path.exists(orc.REF_SO), reason="oracle/_ref/libref.so not built (no reference tree)")
+@pytest.mark.parametrize("case", QUALITY_CASES, ids=[c[0] for c in QUALITY_CASES])
+def test_recompute_quality_oracle_and_engine_equal_the_reference(case):
+    """or_recompute_quality and the engine's Solution::recompute_quality (sankhya_pdhg_quality)
+    against the reference's own compiled Solution::recompute_quality, bit for bit."""
+    name, lp, x, yd, dj, db = case
+    R = orc.ref()
+    P = orc.Problem(lp)
+    act_ref, q_ref = np.zeros(max(P.m, 1)), np.zeros(9)
+    f = lambda v: np.ascontiguousarray(v if len(v) else np.zeros(1), np.float64)
+    R.ref_recompute_quality(P.m, P.n, P.colptr, P.rowidx, f(P.val), f(P.cost), f(P.col_lower),
+                            f(P.col_upper), f(P.row_lower), f(P.row_upper), P.offset,
+                            1 if P.maximize else 0, f(x), f(yd), f(dj), db, act_ref, q_ref)
+    act_or, q_or = orc.recompute_quality(P, f(x), f(yd), f(dj), db)
+    assert np.array_equal(act_or, act_ref[: P.m]), name
+    assert np.array_equal(q_or, q_ref), (name, q_or, q_ref)
+    act_en, q_en = sh.quality(cm.model_of(lp), x, yd, dj, db)
+    assert np.array_equal(act_en, act_ref[: P.m]), name
+    assert np.array_equal(q_en, q_ref), (name, q_en, q_ref)
+
+
+def test_quality_cases_violate_every_kind_of_term():
+    """The points above are not near-optimal: across the set every field the comparison covers
+    is nonzero somewhere, so agreement is not agreement on zeros."""
+    seen = np.zeros(9, dtype=bool)
+    for name, lp, x, yd, dj, db in QUALITY_CASES:
+        _, q = orc.recompute_quality(orc.Problem(lp), x if len(x) else np.zeros(1),
+                                     yd if len(yd) else np.zeros(1), dj if len(dj) else np.zeros(1), db)
+        seen |= q != 0.0
+    assert seen[[0, 1, 2, 3, 4, 5, 7, 8]].all(), seen  # integrality (6) is 0 for an LP
+
+
+def _validate_cases():
+    ok = cm.dense_lp([[1.0, 2.0], [0.0, 3.0]], [1.0, -np.inf], [4.0, 2.0], [1.0, 1.0])
+    cases = [("well-formed", ok)]
+    def edit(name, **kv):
+        lp = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in ok.items()}
+        for k, (idx, val) in kv.items():
+            lp[k][idx] = val
+        cases.append((name, lp))
+    edit("nan-entry", val=(1, np.nan))
+    edit("infinite-entry", val=(0, np.inf))
+    edit("nan-column-bound", col_lower=(1, np.nan))
+    edit("column-lower-above-upper", col_lower=(0, 2.5), col_upper=(0, 1.25))
+    edit("nan-cost", cost=(1, np.nan))
+    edit("nan-row-bound", row_upper=(0, np.nan))
+    edit("row-lower-above-upper", row_lower=(1, 7.5), row_upper=(1, 0.5))
+    return cases
+
+
+VALIDATE_CASES = _validate_cases()
+
+
+@pytest.mark.skipif(not os.path.exists(orc.REF_SO), reason="oracle/_ref/libref.so not built (no reference tree)")
+@pytest.mark.parametrize("case", VALIDATE_CASES, ids=[c[0] for c in VALIDATE_CASES])
+def test_validate_messages_equal_the_reference(case):
+    """The engine's Model::validate (through SANKHYA's C API, sankhya_model_validate) gives the
+    reference's message text for each malformed model, and accepts the well-formed one."""
+    import ctypes as C
+    name, lp = case
+    R = orc.ref()
+    P = orc.Problem(lp)
+    buf = C.create_string_buffer(512)
+    R.ref_validate(P.m, P.n, P.colptr, P.rowidx, P.val, P.cost, P.col_lower, P.col_upper,
+                   P.row_lower, P.row_upper, buf, 512)
+    want = buf.value.decode()
+    lib = sh.load_library()
+    lib.sankhya_model_validate.argtypes = [C.c_void_p]
+    lib.sankhya_last_error.restype = C.c_char_p
+    mdl = cm.model_of(lp)
+    rc = lib.sankhya_model_validate(mdl._h)
+    got = "" if rc == 0 else lib.sankhya_last_error().decode()
+    assert got == want, (name, got, want)
+    assert (name == "well-formed") == (want == "")
