@@ -4,32 +4,40 @@ its dominant kernel and a CPU baseline.
 Workload (default, "zipf-200k-500k-fp64"): the bandwidth-scale synthetic LP, m = 200,000 rows,
 n = 500,000 columns, 10 nonzeros per column at Zipf(1.5)-distributed rows (nnz = 5,000,000;
 152 rows hold 88% of the nonzeros), 70% equality / 30% `<=` rows, x0 ~ U[0,1], bounds [0,10],
-b = A x0, c = A^T y0 + z (seed 2). Plain FP64. Solved = relative KKT error (the max of the
-relative primal, dual and gap residuals the reference computes) <= 1e-6, the metric's
-tolerance (engine option pdhg_stop_rule=relative_kkt). Under the reference's own stopping test
-this instance does not terminate: its absolute-primal clause (<= 1e-7 in the model's units,
-with |b| in the hundreds on the dense rows) is not reached within 1e6 iterations.
+b = A x0, c = A^T y0 + z (seed 2). Plain FP64.
 
-A step is one whole solve of the LP held in HBM (FP64): it is uploaded once before the timed
-region, and each step preconditions it again on the device from the kept values, runs the
-power iteration, the iterations and the final point, with nothing uploaded from the host.
-`pcie_inclusive` gives the same number of whole solve() calls from the host Model (host
-preparation, upload, report included); `resident_check` says whether the resident solve ended
-on the same iteration and point, bit for bit. With --precision mixed the headline keeps the
-whole solve() steps. Instance generation is outside every timed region.
+A step is one whole solve() from the host Model: host preparation (validation, norms, CSR),
+upload over PCIe, device preconditioning, power iteration, the iterations, the report and the
+release; instance generation is outside the timed region. `value` is LPs per second over the
+timed steps. Solved for `value` = relative KKT error (the max of the relative primal, dual and
+gap residuals the reference computes) <= 1e-6, the metric's tolerance (engine option
+pdhg_stop_rule=relative_kkt). That is NOT the reference's "solved" (meets_request: the same
+three <= 1e-6 AND absolute primal <= 1e-7 in the model's units): on this LP the absolute clause
+(|b| in the hundreds on the dense rows) is not met within 1e6 iterations, and the engine then
+reports iteration_limit, claiming no feasibility. `config.absolute_primal` gives the value
+reached. The reference's definition is measured in `reference_rule` on the 1,000 x 3,000 LPs,
+where it is reached.
 
-Beside the headline line's `value`, two measurements for the metric's "mixed/batched vs plain
-FP64" half:
-  - `mixed`: the same LP solved once with FP32 iterate deltas and FP64 ticks (LPs/s, iterations,
-    objective difference from the FP64 solve);
+Other fields of the line:
+  - `resident`: the same solves of the LP held in HBM (uploaded once; each solve preconditions
+    again on the device from the kept values), and whether they end on the same iteration and
+    point bit for bit as the whole solve();
+  - `reference_rule`: the 1,000 x 3,000 uniform-equality LPs (seeds 1-4) solved under the
+    reference's own stopping test at 1e-6 (pdhg_stop_at_request), GPU whole solves, beside the
+    CPU oracle's full solve of seed 1 on one core and on all cores (A^T y gathered on every
+    core, as the reference's threads option does);
+  - `mixed`: the same 200k x 500k LP with FP32 iterate deltas and FP64 ticks;
+  - `c2`: the 1,000,000 x 2,000,000 Zipf LP (nnz 20M, seed 3) in mixed precision to relative
+    KKT 1e-6, then the FP64 polish toward 1e-8 (iteration cap), with the objective error against
+    an FP64 solve of the same LP at 1e-6;
   - `batched`: 512 LPs of the batched workload (m log-uniform in [200, 20000], n = 2.5 m, 6
-    nonzeros per column, relative KKT <= 1e-6) solved in one batch in mixed precision, against
-    plain FP64 single-LP solves of a 16-LP sample of the same LPs (their LPs/s extrapolated to
-    the batch); `speedup` is the ratio of the two rates. The batch runs with the schedule tuner
-    on (the engine's default); `tuner` says how it placed the LPs.
-And `adjoint`: 64 blend-style LPs (m = 5,000, n = 12,000) solved at relative KKT <= 1e-9, then
-the adjoint: forward and backward seconds, and the envelope error of L = c.x* (dL/db against
-y*) over the LPs with a square active system (see adjoint_check for degenerate vertices).
+    nonzeros per column) solved in one batch in mixed precision to relative KKT 1e-6, and again
+    polished to 1e-8 in FP64 (pdhg_batch_promote), each against plain FP64 single-LP solves of a
+    sample of the same LPs at the same tolerance; `speedup` is the ratio of the LPs/s;
+  - `adjoint`: 64 blend-style LPs (m = 5,000, n = 12,000) solved at relative KKT <= 1e-9, then
+    the adjoint: forward and backward seconds, and the envelope error of L = c.x* (dL/db
+    against y*) over the LPs with a square active system;
+  - `roofline`: the dominant kernel's algorithmic bytes over its HIP-event-timed duration.
 
 Prints one JSON line. One GPU only: --gpus above 1 exits with a message.
 """
@@ -114,9 +122,22 @@ def cpu_baseline(arrays, gpu_iterations, tolerance, budget_s=20.0):
                       f"({est:.1f}s per LP); status of the sample run {r0['status']}"}
 
 
-def batched_comparison(sh, count):
-    """Mixed-precision batched solve of `count` LPs of the batched workload, against FP64
-    single-LP solves of a 16-LP sample (every count/16-th LP)."""
+def _single_sample(sh, models, opts, tol):
+    t = time.perf_counter()
+    single = [sh.solve(m, **opts) for m in models]
+    ts = time.perf_counter() - t
+    ok = sum(1 for s in single if s.reached(tol))
+    return ok, ts
+
+
+def batched_comparison(sh, count, polished_sample=4, polished_lps=128, polished_cap=500_000):
+    """Mixed-precision batched solves of `count` LPs of the batched workload to relative KKT
+    1e-6, against FP64 single-LP solves of a 16-LP sample of the same LPs (every count/16-th),
+    whose LPs/s stand for the whole workload. Then the FP64-polished form: the first
+    `polished_lps` LPs in one batch, FP32 phase to 1e-6 and pdhg_batch_promote toward 1e-8, under
+    an iteration cap (PDHG's relative primal residual levels off near 1e-8 on many of these LPs),
+    against FP64 single-LP solves of `polished_sample` of them under the same cap; solved there
+    means relative KKT <= 1e-8."""
     import sankhya_hip.workloads as wl
 
     opts = dict(pdhg_tolerance=1e-6, pdhg_stop_rule="relative_kkt", iteration_limit=1_000_000)
@@ -130,26 +151,141 @@ def batched_comparison(sh, count):
     solved = sum(1 for s in sols if s.reached(1e-6))
     log(f"[bench] batched mixed: {solved}/{count} solved in {tb:.2f}s")
     sample = models[:: max(1, count // 16)]
-    t = time.perf_counter()
-    single = [sh.solve(m, **opts) for m in sample]
-    ts = time.perf_counter() - t
-    single_ok = sum(1 for s in single if s.reached(1e-6))
+    single_ok, ts = _single_sample(sh, sample, opts, 1e-6)
     single_rate = single_ok / ts if ts > 0 else 0.0
     log(f"[bench] fp64 single-LP sample: {single_ok}/{len(sample)} in {ts:.2f}s")
     its = sorted(s.iterations for s in sols)
     persistent = [s.stats["schedule"] for s in sols if s.stats["schedule"] >= 0]
-    return {"workload": f"batched-{count}: m log-uniform [200, 20000], n = 2.5 m, 6 nnz/col, "
-                        "relative KKT <= 1e-6",
-            "value": solved / tb, "unit": "LPs/s", "precision": "mixed", "seconds": tb,
-            "tuner": {"cooperative": count - len(persistent),
-                      "persistent_by_geometry": [persistent.count(g) for g in range(3)],
-                      "seconds_outside_the_loop": (sols[0].solve_seconds - sols[0].stats["loop_seconds"])
-                      if sols else None},
-            "solved": solved, "lps": count, "max_iterations": its[-1],
-            "median_iterations": its[len(its) // 2],
-            "fp64_single_value": single_rate, "fp64_single_sample": len(sample),
-            "fp64_single_seconds": ts,
-            "speedup": (solved / tb) / single_rate if single_rate > 0 else None}
+    out = {"workload": f"batched-{count}: m log-uniform [200, 20000], n = 2.5 m, 6 nnz/col, "
+                       "relative KKT <= 1e-6",
+           "value": solved / tb, "unit": "LPs/s", "precision": "mixed", "seconds": tb,
+           "tuner": {"cooperative": count - len(persistent),
+                     "persistent_by_geometry": [persistent.count(g) for g in range(3)],
+                     "seconds_outside_the_loop": (sols[0].solve_seconds - sols[0].stats["loop_seconds"])
+                     if sols else None},
+           "solved": solved, "lps": count, "max_iterations": its[-1],
+           "median_iterations": its[len(its) // 2],
+           "fp64_single_value": single_rate, "fp64_single_sample": len(sample),
+           "fp64_single_seconds": ts,
+           "speedup": (solved / tb) / single_rate if single_rate > 0 else None}
+    if polished_sample > 0 and polished_lps > 0:
+        popts = dict(opts, pdhg_tolerance=1e-8, iteration_limit=polished_cap)
+        pmodels = models[:polished_lps]
+        t = time.perf_counter()
+        psols = sh.solve_batch(pmodels, pdhg_precision="mixed", **popts)
+        tp = time.perf_counter() - t
+        psolved = sum(1 for s in psols if s.reached(1e-8))
+        fp32 = sorted(s.stats["fp32_iterations"] for s in psols)
+        pits = sorted(s.iterations for s in psols)
+        log(f"[bench] batched mixed, FP64-polished toward 1e-8: {psolved}/{len(pmodels)} in {tp:.2f}s")
+        psample = pmodels[:: max(1, len(pmodels) // polished_sample)]
+        pok, t = 0, time.perf_counter()
+        for k, md in enumerate(psample):
+            s1 = sh.solve(md, **popts)
+            pok += 1 if s1.reached(1e-8) else 0
+            log(f"[bench] fp64 single LP {k} at 1e-8: {s1.status} kkt {s1.relative_kkt:.1e} "
+                f"it={s1.iterations}")
+        pts = time.perf_counter() - t
+        prate = pok / pts if pts > 0 else 0.0
+        log(f"[bench] fp64 single-LP sample at 1e-8: {pok}/{len(psample)} in {pts:.2f}s")
+        out["polished_1e8"] = {
+            "what": f"the first {len(pmodels)} LPs: FP32 phase to relative KKT 1e-6, then every LP "
+                    f"continued in FP64 toward 1e-8, at most {polished_cap} iterations; solved = "
+                    "relative KKT <= 1e-8",
+            "value": psolved / tp, "unit": "LPs/s", "seconds": tp, "solved": psolved,
+            "lps": len(pmodels), "iteration_cap": polished_cap,
+            "median_iterations": pits[len(pits) // 2], "max_iterations": pits[-1],
+            "median_fp32_iterations": fp32[len(fp32) // 2],
+            "fp64_single_value": prate, "fp64_single_sample": len(psample),
+            "fp64_single_seconds": pts,
+            "speedup": (psolved / tp) / prate if prate > 0 else None}
+    return out
+
+
+def reference_rule_check(sh, seeds=(1, 2, 3, 4), cpu=True):
+    """The reference's own definition of solved (meets_request at 1e-6 with
+    pdhg_stop_at_request: relative primal, dual, gap <= 1e-6 and absolute primal <= 1e-7) on the
+    1,000 x 3,000 uniform-equality LPs: GPU whole solves of each seed, and the CPU oracle's full
+    solve of the first seed on one core and on all cores."""
+    opts = dict(pdhg_tolerance=1e-6, pdhg_stop_at_request=True, pdhg_polish=False,
+                iteration_limit=1_000_000)
+    models = [sh.Model.generate("uniform", 1000, 3000, 5, s) for s in seeds]
+    for md in models[:1]:
+        sh.solve(md, **dict(opts, iteration_limit=1000))  # warm-up (graph capture paths)
+    t = time.perf_counter()
+    sols = []
+    for sd, md in zip(seeds, models):
+        sols.append(sh.solve(md, **opts))
+        log(f"[bench] reference rule seed {sd}: {sols[-1].status} it={sols[-1].iterations}")
+    tg = time.perf_counter() - t
+    solved = sum(1 for s in sols if s.status in ("optimal", "feasible"))
+    log(f"[bench] reference rule, 1000x3000 seeds {list(seeds)}: {solved}/{len(seeds)} solved in "
+        f"{tg:.2f}s on the GPU; iterations {[s.iterations for s in sols]}")
+    out = {"workload": "uniform-1k-3k: m = 1,000 equality rows, n = 3,000, 5 nnz/col, seeds "
+                       f"{list(seeds)}, the reference's test at 1e-6 (stop at request)",
+           "value": solved / tg, "unit": "LPs/s", "seconds": tg, "solved": solved,
+           "statuses": [s.status for s in sols], "iterations": [s.iterations for s in sols],
+           "objectives": [s.objective for s in sols]}
+    if cpu:
+        import oracle as orc
+
+        P = orc.Problem(models[0].arrays())
+        legs = {}
+        for label, threads in (("one_core", 1), ("all_cores", os.cpu_count() or 1)):
+            orc.set_threads(threads)
+            t = time.perf_counter()
+            r = orc.solve(P, tolerance=1e-6, stop_at_request=True, iteration_limit=1_000_000)
+            tc = time.perf_counter() - t
+            legs[label] = {"value": (1.0 / tc) if r["status"] in ("optimal", "feasible") else 0.0,
+                           "unit": "LPs/s", "seconds": tc, "threads": threads, "status": r["status"],
+                           "iterations": r["iterations"], "objective": r["objective"]}
+            log(f"[bench] oracle seed {seeds[0]} on {threads} thread(s): {r['status']} "
+                f"it={r['iterations']} {tc:.2f}s")
+        orc.set_threads(1)
+        g = sols[0]
+        legs["gpu_vs_oracle_seed"] = {
+            "status_equal": g.status == legs["one_core"]["status"],
+            "objective_rel_diff": abs(g.objective - legs["one_core"]["objective"]) /
+                                  max(1.0, abs(legs["one_core"]["objective"]))}
+        out["cpu_oracle"] = legs
+    return out
+
+
+def c2_check(sh, iteration_cap=100_000):
+    """The 1,000,000 x 2,000,000 Zipf LP (nnz 20M, seed 3): mixed precision, FP32 phase to
+    relative KKT 1e-6, then the FP64 polish toward 1e-8 under an iteration cap; and a plain FP64
+    solve to 1e-6 whose objective the mixed result is compared with."""
+    t = time.perf_counter()
+    model = sh.Model.generate("zipf", 1_000_000, 2_000_000, 10, 3)
+    log(f"[bench] c2: generated 1M x 2M zipf in {time.perf_counter()-t:.1f}s")
+    base = dict(pdhg_stop_rule="relative_kkt")
+    t = time.perf_counter()
+    a = sh.solve(model, **dict(base, pdhg_tolerance=1e-6, iteration_limit=1_000_000))
+    ta = time.perf_counter() - t
+    t = time.perf_counter()
+    b = sh.solve(model, **dict(base, pdhg_tolerance=1e-6, iteration_limit=1_000_000,
+                               pdhg_precision="mixed"))
+    tb = time.perf_counter() - t
+    t = time.perf_counter()
+    c = sh.solve(model, **dict(base, pdhg_tolerance=1e-8, iteration_limit=iteration_cap,
+                               pdhg_precision="mixed"))
+    tc = time.perf_counter() - t
+    rel = lambda s: abs(s.objective - a.objective) / max(1.0, abs(a.objective))
+    log(f"[bench] c2: fp64 1e-6 it={a.iterations} {ta:.2f}s; mixed 1e-6 it={b.iterations} {tb:.2f}s "
+        f"(rel {rel(b):.1e}); mixed+polish it={c.iterations} (fp32 {c.stats['fp32_iterations']}) "
+        f"kkt {c.relative_kkt:.2e} {tc:.2f}s")
+    return {"workload": "zipf-1m-2m: m = 1,000,000, n = 2,000,000, nnz 20M, seed 3",
+            "fp64_1e6": {"seconds": ta, "iterations": a.iterations, "relative_kkt": a.relative_kkt,
+                         "status": a.status},
+            "mixed_1e6": {"seconds": tb, "iterations": b.iterations, "relative_kkt": b.relative_kkt,
+                          "objective_rel_error_vs_fp64": rel(b), "status": b.status,
+                          "speedup_vs_fp64": ta / tb if tb > 0 else None},
+            "mixed_polished_toward_1e8": {
+                "seconds": tc, "iterations": c.iterations, "iteration_cap": iteration_cap,
+                "fp32_iterations": c.stats["fp32_iterations"], "relative_kkt": c.relative_kkt,
+                "relative_primal": c.stats["final_primal"], "relative_dual": c.stats["final_dual"],
+                "relative_gap": c.stats["final_gap"], "reached_1e8": c.reached(1e-8),
+                "objective_rel_diff_vs_fp64_1e6": rel(c), "status": c.status}}
 
 
 def adjoint_check(sh, count):
@@ -224,11 +360,16 @@ def adjoint_check(sh, count):
 
 
 def host_info():
+    """CPU model (lscpu's model name) and logical CPU count (nproc) of this host."""
     try:
         model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
     except Exception:
         model = "unknown"
-    return model, os.cpu_count()
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+    except Exception:
+        nproc = os.cpu_count()
+    return model, nproc
 
 
 def main():
@@ -241,12 +382,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-reps", type=int, default=50)
     ap.add_argument("--iteration-limit", type=int, default=1_000_000,
-                    help="cap per solve (profiling runs); the default is the engine's own ceiling")
+                    help="iteration limit of the headline solves (profiling runs cap it)")
     ap.add_argument("--batched-lps", type=int, default=512,
                     help="LPs in the batched comparison (0 skips it)")
+    ap.add_argument("--polished-sample", type=int, default=4,
+                    help="FP64 single-LP sample of the batch's 1e-8 comparison (0 skips that leg)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision solve")
+    ap.add_argument("--no-reference-rule", action="store_true", help="skip the reference-rule section")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 1M x 2M section")
     ap.add_argument("--adjoint-lps", type=int, default=64,
-                    help="blend LPs for the adjoint measurement (0 skips it)")
+                    help="blend LPs in the adjoint check (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -268,6 +413,7 @@ def main():
     opts = dict(pdhg_tolerance=tol, pdhg_stop_at_request=stop_req, pdhg_stop_rule=rule,
                 iteration_limit=args.iteration_limit, pdhg_precision=args.precision)
 
+    # ---- headline: whole solve() calls, upload included ----
     for w in range(args.warmup):
         s = sh.solve(model, **opts)
         log(f"[bench] warmup {w}: {s.status} obj={s.objective:.12g} it={s.iterations} "
@@ -283,44 +429,38 @@ def main():
             f"restarts={s.stats['restarts']} t={times[-1]:.3f}s (setup {s.stats['setup_seconds']:.3f}s, "
             f"loop {s.stats['loop_seconds']:.3f}s)")
     total = time.perf_counter() - t0
-    # "Solved" = the engine stopped on the requested criterion (status optimal or feasible).
+    # Solved: relative KKT <= tol under the relative_kkt rule (no feasibility claimed when the
+    # absolute primal clause is missed), or the status's claim under the reference's rule.
     solved = sum(1 for s in sols if s.reached(tol))
     value = solved / total if total > 0 else 0.0
     last = sols[-1]
-    pcie = {"value": value, "unit": "LPs/s", "ms_per_solve": 1e3 * total / max(args.steps, 1),
-            "steps": args.steps,
-            "what": "whole solve() from the host Model: host preparation, upload over PCIe, device "
-                    "preconditioning, power iteration, loop, report, device release"}
+
+    # ---- the same LP resident in HBM (secondary) ----
     resident = None
     if args.precision == "fp64" and args.steps > 0:
-        # The headline: the LP resident in HBM, uploaded once before the timed region; each step
-        # is a whole solve on the device (preconditioning from the kept values, power iteration,
-        # loop, final point), nothing uploaded from the host.
         rdev = sh.Device(model)
         ropts = dict(tolerance=tol, stop_rule=1 if rule == "relative_kkt" else 0,
                      stop_at_request=stop_req, iteration_limit=args.iteration_limit)
-        for w in range(args.warmup):
-            st, *_ = rdev.solve(**ropts)
-            log(f"[bench] resident warmup {w}: it={st.iteration} done={st.done}")
-        rsolved, t0 = 0, time.perf_counter()
-        for k in range(args.steps):
-            ts = time.perf_counter()
+        rsteps = min(args.steps, 5)
+        st, *_ = rdev.solve(**ropts)
+        rsolved, t1 = 0, time.perf_counter()
+        for k in range(rsteps):
             st, x, y, _, _ = rdev.solve(**ropts)
             rsolved += 1 if st.done else 0
-            log(f"[bench] resident step {k}: it={st.iteration} restarts={st.restarts} done={st.done} "
-                f"t={time.perf_counter() - ts:.3f}s")
-        rtotal = time.perf_counter() - t0
+        rtotal = time.perf_counter() - t1
         rdev.close()
         same = bool(st.iteration == last.iterations and np.array_equal(x, last.col_value)
                     and np.array_equal(-y, last.row_dual))
-        resident = {"iterations": int(st.iteration), "identical_to_whole_solve": same}
-        log(f"[bench] resident: {rsolved}/{args.steps} in {rtotal:.3f}s; same point as the whole "
+        resident = {"value": rsolved / rtotal if rtotal > 0 else 0.0, "unit": "LPs/s",
+                    "ms_per_solve": 1e3 * rtotal / max(rsteps, 1), "steps": rsteps,
+                    "iterations": int(st.iteration), "identical_to_whole_solve": same,
+                    "what": "the LP held in HBM, uploaded once; each solve preconditions again on "
+                            "the device from the kept values, nothing uploaded from the host"}
+        log(f"[bench] resident: {rsolved}/{rsteps} in {rtotal:.3f}s; same point as the whole "
             f"solve: {same}")
-        value = rsolved / rtotal if rtotal > 0 else 0.0
-        total = rtotal
 
-    # Roofline of the dominant kernel: standalone launches on the engine's stream, timed with
-    # HIP events, on the same uploaded LP.
+    # ---- roofline of the dominant kernel: standalone launches on the engine's stream, timed
+    # with HIP events, on the same uploaded LP ----
     dev = sh.Device(model, precision=args.precision)
     dev.start(tolerance=tol, spectral_norm=last.stats["spectral_norm"],
               stop_rule=1 if rule == "relative_kkt" else 0)
@@ -346,20 +486,29 @@ def main():
                  "fp32_iterations": sm.stats["fp32_iterations"], "seconds": tm,
                  "objective_rel_diff_vs_fp64": abs(sm.objective - last.objective) / max(1.0, abs(last.objective))}
         log(f"[bench] mixed: {sm.status} it={sm.iterations} t={tm:.3f}s")
+
+    reference = None
+    if not args.no_reference_rule:
+        reference = reference_rule_check(sh, cpu=not args.no_cpu_baseline)
+
+    c2 = None
+    if not args.no_c2:
+        c2 = c2_check(sh)
+
     batched = None
     if args.batched_lps > 0:
-        batched = batched_comparison(sh, args.batched_lps)
+        batched = batched_comparison(sh, args.batched_lps, args.polished_sample)
 
     adjoint = None
     if args.adjoint_lps > 0:
         adjoint = adjoint_check(sh, args.adjoint_lps)
 
+    cpu_model, ncpu = host_info()
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline (oracle, one core) on a bounded sample")
         cpu = cpu_baseline(model.arrays(), last.iterations, tol)
-        cpu_model, ncpu = host_info()
-        cpu["sample"] += f"; host {cpu_model}, {ncpu} logical CPUs"
+        cpu["sample"] += f"; host {cpu_model}, nproc {ncpu}"
 
     line = {
         "metric": "LPs solved/sec to 1e-6 rel KKT + SpMV HBM GB/s; mixed/batched vs plain FP64",
@@ -376,17 +525,26 @@ def main():
         "data": "synthetic (deterministic generator, seeded)",
         "config": {"workload": args.workload, "m": M, "n": N, "nnz": NNZ,
                    "tolerance": tol, "stop_rule": rule, "precision": args.precision,
+                   "solved_means": ("relative KKT <= tolerance (pdhg_stop_rule=relative_kkt); not "
+                                    "the reference's meets_request, which also needs absolute "
+                                    "primal <= 1e-7 (see reference_rule)") if rule == "relative_kkt"
+                   else "the reference's meets_request (stop at request)",
+                   "step": "one whole solve() from the host Model, upload over PCIe included",
                    "iterations": last.iterations, "status": last.status,
-                   "objective": last.objective, "restarts": last.stats["restarts"]},
+                   "objective": last.objective, "restarts": last.stats["restarts"],
+                   "relative_kkt": last.relative_kkt,
+                   "absolute_primal": last.stats["final_absolute_primal"]},
         "roofline": {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": traffic_bytes(dom["name"], args),
                      "kernel": dom["name"],
                      "kernel_us": dom["ms"] * 1e3, "bytes_per_launch": dom["bytes"],
                      "other_kernel": {k: v for k, v in (kp if dom is kd else kd).items()}},
-        "pcie_inclusive": pcie,
-        "resident_check": resident,
+        "resident": resident,
+        "reference_rule": reference,
         "cpu_baseline": cpu,
+        "host": {"cpu": cpu_model, "nproc": ncpu},
         "mixed": mixed,
+        "c2": c2,
         "batched": batched,
         "adjoint": adjoint,
     }

@@ -73,6 +73,13 @@ int pdhg_batch_tune(pdhg_batch* h, const pdhg_params* p, const double* spectral_
  * p->chunk_attempts is the attempts each LP runs per graph launch (0: 480). */
 int pdhg_batch_start(pdhg_batch* h, const pdhg_params* p, const double* spectral_norms);
 
+/* Mixed-precision handoff of the whole batch (no reference counterpart), as pdhg_dev_promote
+ * does for one LP: on a PDHG_PRECISION_MIXED batch still in its FP32 phase, widens every LP's
+ * iterate pairs to FP64 exactly, keeps every other piece of each LP's state, installs the new
+ * tolerance, stopping rule and iteration limit from p, clears the stop flags and recaptures the
+ * graph with FP64 values. The next pdhg_batch_run continues every LP in FP64. */
+int pdhg_batch_promote(pdhg_batch* h, const pdhg_params* p);
+
 /* Launches `chunks` graph chunks; *running receives how many LPs have not stopped. */
 int pdhg_batch_run(pdhg_batch* h, int chunks, int32_t* running);
 

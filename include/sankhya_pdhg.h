@@ -58,6 +58,7 @@ typedef struct {
   double objective, dual_bound, solve_seconds, setup_seconds, loop_seconds, spectral_norm;
   double primal_infeasibility, dual_infeasibility, complementarity_violation, relative_gap;
   double final_primal, final_dual, final_gap;
+  double final_absolute_primal; /* ||primal violation||_2 in the model's units at the report */
   char algorithm[32];
   char message[1024];
 } sankhya_result;

@@ -34,7 +34,7 @@ class OrModel(C.Structure):
 class OrOptions(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("iteration_limit", C.c_int64),
                 ("restart", C.c_int32), ("stop_at_request", C.c_int32),
-                ("random_seed", C.c_uint32), ("time_limit", C.c_double)]
+                ("random_seed", C.c_uint32), ("time_limit", C.c_double), ("stop_rule", C.c_int32)]
 
 
 class OrResiduals(C.Structure):
@@ -87,6 +87,7 @@ def lib():
         L.or_solve_pdhg.restype = C.c_int
         L.or_solve_pdhg.argtypes = [C.POINTER(OrModel), C.POINTER(OrOptions), _f64p, _f64p, _f64p,
                                     C.POINTER(OrResult), C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]
+        L.or_set_threads.argtypes = [C.c_int]
         L.or_recompute_quality.argtypes = [C.POINTER(OrModel), _f64p, _f64p, _f64p, C.c_double,
                                            _f64p, C.POINTER(OrResult)]
         L.or_mt19937_seed.argtypes = [C.POINTER(OrMt), C.c_uint32]
@@ -193,12 +194,18 @@ def evaluate(P: Problem, x, y):
     return act[: P.m], red[: P.n], r.as_array()
 
 
+def set_threads(threads: int) -> None:
+    """Threads of the oracle's A^T y gather (the reference's `threads` option); results do not
+    depend on it."""
+    lib().or_set_threads(int(threads))
+
+
 def solve(P: Problem, tolerance=1e-4, iteration_limit=-1, restart=True, stop_at_request=False,
-          random_seed=0, time_limit=0.0, max_ticks=0):
+          random_seed=0, time_limit=0.0, max_ticks=0, stop_rule=0):
     x, y, d = np.zeros(max(P.n, 1)), np.zeros(max(P.m, 1)), np.zeros(max(P.n, 1))
     res = OrResult()
     opts = OrOptions(tolerance, iteration_limit, 1 if restart else 0, 1 if stop_at_request else 0,
-                     random_seed, time_limit)
+                     random_seed, time_limit, stop_rule)
     ticks = (OrTick * max(max_ticks, 1))()
     nt = C.c_int64()
     M = P.c_model()

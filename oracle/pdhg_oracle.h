@@ -65,6 +65,9 @@ typedef struct {
   int32_t stop_at_request; /* pdhg_stop_at_request */
   uint32_t random_seed;    /* the power iteration uses random_seed + 1 */
   double time_limit;       /* seconds, <=0 or inf: none */
+  int32_t stop_rule;       /* 0: the reference's test; 1: the engine's pdhg_stop_rule=relative_kkt
+                              (max of relative primal, dual, gap <= tolerance), for comparisons
+                              with the engine run that way */
 } or_options;
 
 /* Residuals as in src/pdhg/pdhg_evaluate.hpp:19-46. */
@@ -98,6 +101,10 @@ typedef struct {
 } or_tick;
 
 int or_is_finite_bound(double v);
+
+/* Threads for the A^T y gather (the reference's `threads` option, src/la/sparse.cpp:248-250);
+ * 1 by default; results do not depend on it. */
+void or_set_threads(int threads);
 
 void or_csc_multiply(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
                      const double* val, const double* x, double* y);

@@ -298,6 +298,17 @@ __global__ void k_batch_init(Iter<X> it) {
     init_row(it, i);
 }
 
+// Exact FP32 -> FP64 widening of every LP's iterate pairs (the mixed-precision handoff):
+// x = xrest + dx for each half of each pair.
+__global__ void k_batch_widen(const float* xs, const double* xr, double* xd, int64_t nx,
+                              const float* ys, const double* yr, double* yd, int64_t ny) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nx; k += stride)
+    xd[k] = xr[k >> 1] + static_cast<double>(xs[k]);
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ny; k += stride)
+    yd[k] = yr[k >> 1] + static_cast<double>(ys[k]);
+}
+
 // ---- the cooperative schedule ------------------------------------------------------------
 // Every cooperative LP's work items (geometry 0 plans: 256 threads, tiles of 2048) form one
 // grid per product, so each attempt of every such LP is three launches shared by all of them:
@@ -854,6 +865,7 @@ using namespace sankhya_hip::batch;
 struct pdhg_batch {
   int device = 0;
   int precision = PDHG_PRECISION_FP64;
+  int arith = PDHG_PRECISION_FP64;  // the iterates' current precision (mixed: FP32 until promoted)
   hipStream_t stream = nullptr;
   hipStream_t aux[kGeos] = {nullptr, nullptr, nullptr};
   hipEvent_t fork = nullptr, join[kGeos] = {nullptr, nullptr, nullptr};
@@ -946,7 +958,8 @@ int bupload(pdhg_batch* h, T** dst, const T* src, size_t count) {
   return PDHG_OK;
 }
 
-bool mixed(const pdhg_batch* h) { return h->precision == PDHG_PRECISION_MIXED; }
+// True while the iterates are FP32 (a mixed batch before pdhg_batch_promote).
+bool mixed(const pdhg_batch* h) { return h->arith == PDHG_PRECISION_MIXED; }
 
 template <typename V>
 Mat<V> mat(const int32_t* rp, const int32_t* col, const V* val) {
@@ -1285,6 +1298,7 @@ int check(pdhg_batch* h, bool need_start) {
 // Control blocks at the start point (src/pdhg/pdhg.cpp:137-192), iterates initialised, and the
 // delta form's anchors.
 int start_state(pdhg_batch* h, const pdhg_params* p, const double* norms) {
+  h->arith = h->precision;  // a mixed batch starts (again) in its FP32 phase
   BHIP_TRY(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(Ctl) * h->nlp, hipMemcpyDeviceToHost,
                              h->stream));
   BHIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -1341,6 +1355,7 @@ int pdhg_batch_create(int device, int precision, pdhg_batch** out) {
   pdhg_batch* h = new pdhg_batch();
   h->device = device;
   h->precision = precision;
+  h->arith = precision;
   bool ok = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreateWithFlags(&h->fork, hipEventDisableTiming) == hipSuccess;
   for (int g = 0; ok && g < kGeos; ++g)
@@ -1411,7 +1426,7 @@ int pdhg_batch_upload(pdhg_batch* h, int32_t nlp, const int32_t* m, const int32_
         (rc = bupload(h, &h->a_rowptr, a_rowptr, static_cast<size_t>(arp))) ||
         (rc = bupload(h, &h->t_rowptr, at_rowptr, static_cast<size_t>(trp))))
       return rc;
-    if (mixed(h)) {
+    if (h->precision == PDHG_PRECISION_MIXED) {
       std::vector<float> af(total), tf(total);
       for (size_t q = 0; q < total; ++q) {
         af[q] = static_cast<float>(L.a_val[q]);
@@ -1438,7 +1453,7 @@ int pdhg_batch_upload(pdhg_batch* h, int32_t nlp, const int32_t* m, const int32_
   int rc;
   if ((rc = bupload(h, &h->desc, h->desc_host.data(), h->desc_host.size()))) return rc;
   const size_t un = static_cast<size_t>(sn), um = static_cast<size_t>(sm);
-  if (mixed(h)) {
+  if (h->precision == PDHG_PRECISION_MIXED) {
     if ((rc = balloc(h, &h->xb32, 2 * un)) || (rc = balloc(h, &h->yb32, 2 * um))) return rc;
   } else if ((rc = balloc(h, &h->xb64, 2 * un)) || (rc = balloc(h, &h->yb64, 2 * um))) {
     return rc;
@@ -1619,6 +1634,48 @@ int pdhg_batch_tune(pdhg_batch* h, const pdhg_params* p, const double* spectral_
     h->exec = nullptr;
   }
   return PDHG_OK;
+}
+
+int pdhg_batch_promote(pdhg_batch* h, const pdhg_params* p) {
+  int rc = check(h, true);
+  if (rc) return rc;
+  if (!p || p->iteration_limit < 0) return bfail(h, PDHG_E_ARGUMENT, "bad parameters");
+  if (h->precision != PDHG_PRECISION_MIXED || !mixed(h))
+    return bfail(h, PDHG_E_STATE, "promote needs a mixed-precision batch still in its FP32 phase");
+  BHIP_TRY(h, hipSetDevice(h->device));
+  const size_t un = static_cast<size_t>(h->sum_n), um = static_cast<size_t>(h->sum_m);
+  if (!h->xb64) {
+    if ((rc = balloc(h, &h->xb64, 2 * un)) || (rc = balloc(h, &h->yb64, 2 * um))) return rc;
+  }
+  const int64_t w = std::max<int64_t>(std::max<int64_t>(2 * h->sum_n, 2 * h->sum_m), 1);
+  const int grid = static_cast<int>(std::min<int64_t>((w + 255) / 256, 8192));
+  k_batch_widen<<<grid, 256, 0, h->stream>>>(h->xb32, h->xrest, h->xb64, 2 * h->sum_n, h->yb32,
+                                            h->yrest, h->yb64, 2 * h->sum_m);
+  BHIP_TRY(h, hipGetLastError());
+  BHIP_TRY(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(Ctl) * h->nlp, hipMemcpyDeviceToHost, h->stream));
+  BHIP_TRY(h, hipStreamSynchronize(h->stream));
+  for (int32_t k = 0; k < h->nlp; ++k) {
+    Ctl& c = h->ctl_host[k];
+    c.tolerance = p->tolerance;
+    c.stop_at_request = p->stop_at_request ? 1 : 0;
+    c.stop_rule = p->stop_rule == 1 ? 1 : 0;
+    c.iteration_limit = p->iteration_limit;
+    c.done = 0;
+    c.limit_hit = 0;
+    c.halt = 0;
+    if (c.iteration >= c.iteration_limit) {
+      c.halt = 1;
+      c.limit_hit = 1;
+    }
+  }
+  BHIP_TRY(h, hipMemcpyAsync(h->ctl, h->ctl_host, sizeof(Ctl) * h->nlp, hipMemcpyHostToDevice, h->stream));
+  BHIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->arith = PDHG_PRECISION_FP64;
+  // Every LP runs again: the cooperative grids take all of them, then the graph is captured
+  // with FP64 values.
+  if ((rc = compact(h, true))) return rc;
+  h->captured_running = h->n_act_lp;
+  return capture(h);
 }
 
 int pdhg_batch_run(pdhg_batch* h, int chunks, int32_t* running) {

@@ -79,13 +79,20 @@ __global__ __launch_bounds__(kTail) void k_primal_tail(PlanView<V> T, Iter<X> it
   if (threadIdx.x == 0) lr_sum[0] = acc.s[0];
 }
 
+// stepf (2 doubles): the step rule's factors for this attempt, evaluated by workgroup 0 while the
+// others work, and read by k_dual_tail.
 template <typename V, typename X>
 __global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ctl* ctl,
-                                                   double* part, int force) {
+                                                   double* part, double* stepf, int force) {
   __shared__ double lds[kTile];
   __shared__ double red[4 * 2];
   const int4 item = A.items[blockIdx.x];
   if (!force && ctl->halt) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const StepFactors f = step_factors(ctl->iteration);
+    stepf[0] = f.shrink;
+    stepf[1] = f.grow;
+  }
   DualGather<X> gather{it.xbuf, ctl->cur};
   DualEpi<X> epi = dual_epi(it, ctl);
   Acc<2, 0> acc;
@@ -104,13 +111,19 @@ template <typename V, typename X>
 __global__ __launch_bounds__(kTail) void k_dual_tail(PlanView<V> A, Iter<X> it, Ctl* ctl,
                                                      const double* part_p, int np,
                                                      const double* lr_p, int has_lr_p,
-                                                     const double* part_d, int nd, int force) {
+                                                     const double* part_d, int nd,
+                                                     const double* stepf, int force) {
   __shared__ double lds[2 * kChunk];
   __shared__ double red[16 * 5];
   if (!force && ctl->halt) return;
   DualEpi<X> epi = dual_epi(it, ctl);
   StepSnap snap{};
-  if (threadIdx.x == 0) snap = step_snap(ctl);
+  StepFactors f{};
+  if (threadIdx.x == 0) {
+    snap = step_snap(ctl);
+    // Workgroup 0 of k_dual wrote them; with no dual items they are evaluated here.
+    f = nd > 0 ? StepFactors{stepf[0], stepf[1]} : step_factors(snap.iteration);
+  }
   // s[0] primal movement, s[1] dual movement, s[2] interaction (per-workgroup partials);
   // s[3], s[4] the same two dual terms from the long rows.
   Acc<5, 0> a;
@@ -129,7 +142,7 @@ __global__ __launch_bounds__(kTail) void k_dual_tail(PlanView<V> A, Iter<X> it, 
   a.s[3] = lacc.s[0];
   a.s[4] = lacc.s[1];
   block_reduce(a, red);
-  if (threadIdx.x == 0) step_decide(ctl, snap, a.s[0], a.s[1] + a.s[3], a.s[2] + a.s[4]);
+  if (threadIdx.x == 0) step_decide(ctl, snap, a.s[0], a.s[1] + a.s[3], a.s[2] + a.s[4], f);
 }
 
 template <typename V, typename X>
@@ -727,6 +740,7 @@ struct pdhg_dev {
   double *dc = nullptr, *dr = nullptr, *cost = nullptr, *cl = nullptr, *cu = nullptr,
          *rl = nullptr, *ru = nullptr;
   double *part_p = nullptr, *lr_p = nullptr, *part_d = nullptr;
+  double* stepf = nullptr;  // the step rule's two factors of the running attempt
   double *part_er = nullptr, *part_ec = nullptr, *lr_sum = nullptr;
   Ctl* ctl = nullptr;
   double* sc_part = nullptr;  // preconditioner scratch: per-workgroup extremes and the result
@@ -879,10 +893,10 @@ void launch_attempt(pdhg_dev* h, int force) {
     k_primal_tail<V, X><<<1, kTail, 0, h->stream>>>(h->T.view<V>(), it, h->ctl, h->lr_p, force);
   if (h->A.nitems > 0)
     k_dual<V, X><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<V>(), it, h->ctl, h->part_d,
-                                                          force);
+                                                          h->stepf, force);
   k_dual_tail<V, X><<<1, kTail, 0, h->stream>>>(h->A.view<V>(), it, h->ctl, h->part_p, h->T.nitems,
                                                 h->lr_p, h->T.nlr > 0 ? 1 : 0, h->part_d,
-                                                h->A.nitems, force);
+                                                h->A.nitems, h->stepf, force);
 }
 
 // Anchor products Ahat xrest and Ahat^T yrest in FP64 (delta form), gated on a restart.
@@ -1148,7 +1162,7 @@ int64_t pdhg_dev_estimate_bytes(int32_t m, int32_t n, int64_t nnz, const int32_t
   total += 7 * bytes(n, 8) + 5 * bytes(m, 8);                         // problem vectors
   const int64_t ia = std::max(items_a, 1), it = std::max(items_t, 1);
   total += bytes(it, 8) + bytes(1, 8) + bytes(2 * ia, 8) + bytes(1, 8) + bytes(9 * ia, 8) + bytes(9 * it, 8);
-  total += bytes(1, sizeof(Ctl)) + bytes(kTickCap, sizeof(pdhg_tick));
+  total += bytes(1, sizeof(Ctl)) + bytes(kTickCap, sizeof(pdhg_tick)) + bytes(2, 8);
   if (unscaled) total += bytes(2 * 1024 + 2, 8);  // preconditioner scratch
   return total;
 }
@@ -1226,7 +1240,8 @@ int pdhg_dev_upload(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz, const int32_
   if ((rc = dalloc(h, &h->part_p, it)) || (rc = dalloc(h, &h->lr_p, 1)) ||
       (rc = dalloc(h, &h->part_d, 2 * ia)) || (rc = dalloc(h, &h->lr_sum, 1)) ||
       (rc = dalloc(h, &h->part_er, 9 * ia)) || (rc = dalloc(h, &h->part_ec, 9 * it)) ||
-      (rc = dalloc(h, &h->ctl, 1)) || (rc = dalloc(h, &h->ticks, static_cast<size_t>(kTickCap))))
+      (rc = dalloc(h, &h->ctl, 1)) || (rc = dalloc(h, &h->ticks, static_cast<size_t>(kTickCap))) ||
+      (rc = dalloc(h, &h->stepf, 2)))
     return rc;
   Ctl c;
   std::memset(&c, 0, sizeof(c));
@@ -1575,8 +1590,8 @@ int pdhg_dev_time_kernels(pdhg_dev* h, int reps, double* primal_ms, double* dual
   }
   HIP_TRY(h, hipEventRecord(e1, h->stream));
   for (int r = 0; r < reps && h->A.nitems > 0; ++r) {
-    if (mixed(h)) k_dual<float, float><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<float>(), itf, h->ctl, h->part_d, 1);
-    else k_dual<double, double><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<double>(), itd, h->ctl, h->part_d, 1);
+    if (mixed(h)) k_dual<float, float><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<float>(), itf, h->ctl, h->part_d, h->stepf, 1);
+    else k_dual<double, double><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<double>(), itd, h->ctl, h->part_d, h->stepf, 1);
   }
   HIP_TRY(h, hipEventRecord(e2, h->stream));
   HIP_TRY(h, hipEventSynchronize(e2));

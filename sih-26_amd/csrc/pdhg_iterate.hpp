@@ -440,16 +440,28 @@ __device__ inline StepSnap step_snap(const Ctl* ctl) {
                   ctl->eta, ctl->eta_ceiling, ctl->cur};
 }
 
-// Adaptive step (src/pdhg/pdhg.cpp:293-335) and the tick test (:341-342), one thread.
-__device__ void step_decide(Ctl* ctl, const StepSnap& sn, double mx, double my, double inter) {
+// The step rule's factors 1 - k^-0.3 and 1 + k^-0.6 for k = max(2, iteration + 1)
+// (src/pdhg/pdhg.cpp:303-306). They depend on the iteration count alone, so a kernel that runs
+// before the decision can evaluate them (two double pow calls) off the decision's critical path.
+struct StepFactors {
+  double shrink, grow;
+};
+__device__ inline StepFactors step_factors(int64_t iteration) {
+  const double exponent = static_cast<double>(iteration + 1 > 2 ? iteration + 1 : 2);
+  return StepFactors{1.0 - pow(exponent, -0.3), 1.0 + pow(exponent, -0.6)};
+}
+
+// Adaptive step (src/pdhg/pdhg.cpp:293-335) and the tick test (:341-342), one thread, with the
+// factors of step_factors(sn.iteration).
+__device__ void step_decide(Ctl* ctl, const StepSnap& sn, double mx, double my, double inter,
+                            const StepFactors& f) {
   const double movement = mx + my;
   const double interaction = fabs(inter);
   const bool no_info = interaction <= 0.0;
   const double limit = no_info ? __builtin_huge_val() : movement / interaction;
   const int64_t iteration = sn.iteration;
-  const double exponent = static_cast<double>(iteration + 1 > 2 ? iteration + 1 : 2);
-  const double shrink = 1.0 - pow(exponent, -0.3);
-  const double grow = 1.0 + pow(exponent, -0.6);
+  const double shrink = f.shrink;
+  const double grow = f.grow;
   const double proposed = smin(shrink * limit, grow * sn.eta);
   int64_t it = iteration;
   if (sn.eta <= limit) {
@@ -477,6 +489,10 @@ __device__ void step_decide(Ctl* ctl, const StepSnap& sn, double mx, double my, 
     ctl->halt = 1;
     ctl->limit_hit = 1;
   }
+}
+
+__device__ inline void step_decide(Ctl* ctl, const StepSnap& sn, double mx, double my, double inter) {
+  step_decide(ctl, sn, mx, my, inter, step_factors(sn.iteration));
 }
 
 // Best tracking, stopping test, restart and primal weight (src/pdhg/pdhg.cpp:344-483) from a

@@ -43,6 +43,7 @@ class Result(C.Structure):
         ("primal_infeasibility", C.c_double), ("dual_infeasibility", C.c_double),
         ("complementarity_violation", C.c_double), ("relative_gap", C.c_double),
         ("final_primal", C.c_double), ("final_dual", C.c_double), ("final_gap", C.c_double),
+        ("final_absolute_primal", C.c_double),
         ("algorithm", C.c_char * 32), ("message", C.c_char * 1024),
     ]
 
@@ -294,7 +295,7 @@ def solve(model: Model, **options) -> Solution:
         raise ValueError(res.message.decode())
     stats = {k: getattr(res, k) for k in ("attempts", "restarts", "ticks", "device_bytes", "fp32_iterations",
                                           "setup_seconds", "loop_seconds", "spectral_norm",
-                                          "final_primal", "final_dual", "final_gap")}
+                                          "final_primal", "final_dual", "final_gap", "final_absolute_primal")}
     return Solution(STATUS_NAMES[res.status], res.objective, res.dual_bound, res.iterations,
                     x[:n], y[:m], d[:n], act[:m], res.algorithm.decode(), res.message.decode(),
                     res.solve_seconds, res.primal_infeasibility, res.dual_infeasibility,
@@ -328,7 +329,7 @@ def write_solution(model: Model, solution: Solution, path, **options) -> None:
 def _solution_of(res, x, y, d, act, stats_extra=None) -> Solution:
     stats = {k: getattr(res, k) for k in ("attempts", "restarts", "ticks", "device_bytes", "fp32_iterations",
                                           "setup_seconds", "loop_seconds", "spectral_norm",
-                                          "final_primal", "final_dual", "final_gap", "schedule")}
+                                          "final_primal", "final_dual", "final_gap", "final_absolute_primal", "schedule")}
     return Solution(STATUS_NAMES[res.status], res.objective, res.dual_bound, res.iterations,
                     x, y, d, act, res.algorithm.decode(), res.message.decode(),
                     res.solve_seconds, res.primal_infeasibility, res.dual_infeasibility,

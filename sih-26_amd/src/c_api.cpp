@@ -182,6 +182,7 @@ int sankhya_pdhg_solve(const sankhya_model* h, const char* options, double* col_
     out->final_primal = st.final_primal;
     out->final_dual = st.final_dual;
     out->final_gap = st.final_gap;
+    out->final_absolute_primal = st.final_absolute_primal;
     copy_text(out->algorithm, sizeof(out->algorithm), s.algorithm);
     copy_text(out->message, sizeof(out->message), s.message);
   }
@@ -255,7 +256,9 @@ sankhya_batch* sankhya_batch_solve(const sankhya_model* const* models, int32_t n
       out->final_primal = st.final_primal;
       out->final_dual = st.final_dual;
       out->final_gap = st.final_gap;
+      out->final_absolute_primal = st.final_absolute_primal;
       out->schedule = st.schedule;
+      out->fp32_iterations = st.fp32_iterations;
     }
     out->objective = s.objective;
     out->dual_bound = s.dual_bound;

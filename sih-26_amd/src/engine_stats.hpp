@@ -20,6 +20,7 @@ struct EngineStats {
   double spectral_norm = 0.0;
   double setup_seconds = 0.0, loop_seconds = 0.0, total_seconds = 0.0;
   double final_primal = 0.0, final_dual = 0.0, final_gap = 0.0;
+  double final_absolute_primal = 0.0;
 };
 
 const EngineStats& last_engine_stats();

@@ -140,12 +140,38 @@ std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models
     }
     logger.info("Schedule: {} cooperative, {} persistent at 256 threads, {} at 512, {} at 1024", coop, count[0], count[1], count[2]);
   }
+  // Mixed precision with the FP64 polish (pdhg_mixed_polish), as the single-LP engine runs it:
+  // the FP32 phase stops each LP on the relative KKT error at max(pdhg_mixed_switch, tolerance);
+  // when every LP has stopped, the whole batch continues in FP64 under the requested rule and
+  // tolerance (pdhg_batch_promote). No handoff is needed when the request is a relative KKT
+  // error no tighter than the switch.
+  const pdhg_params final_params = params;
+  bool fp32_phase = mixed && options.get_bool("pdhg_mixed_polish");
+  if (fp32_phase) {
+    params.tolerance = std::max(tolerance, options.get_double("pdhg_mixed_switch"));
+    params.stop_rule = 1;
+    if (final_params.stop_rule == 1 && params.tolerance <= tolerance) fp32_phase = false;
+  }
   rc = pdhg_batch_start(dev_h, &params, norms.data());
   if (rc != PDHG_OK) return dev_error(rc, "start");
   const double loop_start = timer.elapsed_seconds();
   SolveStatus stop_status = SolveStatus::kIterationLimit;
   int32_t running = static_cast<int32_t>(nb);
-  while (running > 0) {
+  std::vector<std::int64_t> fp32_iterations(nb, 0);
+  while (running > 0 || fp32_phase) {
+    if (running == 0 && fp32_phase) {
+      std::vector<pdhg_state> st32(nb);
+      rc = pdhg_batch_states(dev_h, st32.data());
+      if (rc != PDHG_OK) return dev_error(rc, "state read");
+      for (std::size_t q = 0; q < nb; ++q) fp32_iterations[q] = st32[q].iteration;
+      logger.info("batch: FP32 phase ended at {:.3f}s; continuing every LP in FP64",
+                  timer.elapsed_seconds() - loop_start);
+      rc = pdhg_batch_promote(dev_h, &final_params);
+      if (rc != PDHG_OK) return dev_error(rc, "FP64 handoff");
+      fp32_phase = false;
+      running = static_cast<int32_t>(nb);
+      continue;
+    }
     if (std::isfinite(time_limit) && timer.elapsed_seconds() >= time_limit) {
       stop_status = SolveStatus::kTimeLimit;
       break;
@@ -188,6 +214,7 @@ std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models
     s.solve_seconds = total;
     EngineStats& es = g_batch_stats[k];
     es.iterations = st.iteration;
+    es.fp32_iterations = mixed ? (fp32_iterations[q] > 0 ? fp32_iterations[q] : st.iteration) : 0;
     es.attempts = st.attempts;
     es.restarts = st.restarts;
     es.ticks = st.ticks;
@@ -199,6 +226,7 @@ std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models
     es.final_primal = res[9 * q + 0];
     es.final_dual = res[9 * q + 1];
     es.final_gap = res[9 * q + 2];
+    es.final_absolute_primal = res[9 * q + 5];
     es.schedule = geo[q];
     report(*models[k], res.data() + 9 * q, yl, rl2,
            // An LP that had already stopped on its own iteration limit keeps that status when
@@ -207,7 +235,7 @@ std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models
                        st.done        ? SolveStatus::kOptimal
                        : st.limit_hit ? SolveStatus::kIterationLimit
                                       : stop_status,
-                       st.iteration, st.restarts, tolerance, stop_at_request, params.stop_rule == 1},
+                       st.iteration, st.restarts, tolerance, stop_at_request, final_params.stop_rule == 1},
            &s);
     off_n += n[q];
     off_m += m[q];

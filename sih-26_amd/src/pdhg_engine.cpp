@@ -276,6 +276,7 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   g_last_stats.final_primal = final_r.primal;
   g_last_stats.final_dual = final_r.dual;
   g_last_stats.final_gap = final_r.gap;
+  g_last_stats.final_absolute_primal = final_r.absolute_primal;
   solution.solve_seconds = timer.elapsed_seconds();
   report(model, res, y, reduced,
          ReportState{converged, stop_status, st.iteration, st.restarts, tolerance, stop_at_request,

@@ -164,6 +164,7 @@ def test_device_memory_estimate_against_a_hand_count():
     want = (plan(m) + plan(n) + _bytes(2 * n, 8) + _bytes(2 * m, 8) + 6 * _bytes(n, 8) + 6 * _bytes(m, 8)
             + 7 * _bytes(n, 8) + 5 * _bytes(m, 8) + _bytes(1, 8) + _bytes(1, 8) + _bytes(2, 8)
             + _bytes(1, 8) + _bytes(9, 8) + _bytes(9, 8) + _bytes(1, ctl) + _bytes(1 << 16, tick)
+            + _bytes(2, 8)
             + _bytes(2 * 1024 + 2, 8))
     got = L.pdhg_dev_estimate_bytes(m, n, nnz, csr["rowptr"].ctypes.data, csr["colidx"].ctypes.data,
                                     d["colptr"].ctypes.data, d["rowidx"].ctypes.data, 0, 1, 0)

@@ -179,3 +179,30 @@ def test_mixed_batch_sample_of_the_batched_workload_against_fp64_single_solves()
         assert rel <= 1e-5, (md.dims, s.objective, r.objective)
     print(f"24 LPs: mixed batch {tb:.2f}s, fp64 one by one {ts:.2f}s ({ts/tb:.1f}x); worst "
           f"relative objective difference {worst:.1e}", flush=True)
+
+
+def test_mixed_batch_polished_toward_1e8_matches_the_exact_optimum():
+    """The batch's FP64 polish (pdhg_batch_promote): three 1,000 x 3,000 LPs in mixed precision,
+    the FP32 phase to relative KKT 1e-6, then every LP continued in FP64 toward 1e-8 (at most
+    1,000,000 iterations). Each LP: its FP32 phase ended before its last iteration, the FP64 phase
+    took the relative KKT error at least 40 times below the handoff's 1e-6 (to about 1e-8; PDHG's
+    relative primal residual levels off near 1e-8 on these LPs, in FP64 alone as in the oracle),
+    and the objective is within 1e-7 relative of the exact optimum (a simplex vertex, HiGHS
+    through scipy), where the FP32 phase alone is not."""
+    pairs = [cm.c0(s) for s in (1, 2, 4)]
+    t = time.time()
+    sols = sh.solve_batch([p[0] for p in pairs], pdhg_precision="mixed", pdhg_tolerance=1e-8,
+                          pdhg_stop_rule="relative_kkt", iteration_limit=1000000)
+    print(f"mixed batch polished toward 1e-8 in {time.time()-t:.2f}s", flush=True)
+    fp32 = sh.solve_batch([p[0] for p in pairs], pdhg_precision="mixed", pdhg_tolerance=1e-6,
+                          pdhg_stop_rule="relative_kkt", iteration_limit=1000000)
+    for (mdl, d), s, f in zip(pairs, sols, fp32):
+        exact = cm.simplex_objective(d)
+        rel = abs(s.objective - exact) / max(1.0, abs(exact))
+        rel32 = abs(f.objective - exact) / max(1.0, abs(exact))
+        print(f"  {s.status} obj={s.objective:.12g} it={s.iterations} (fp32 {s.stats['fp32_iterations']}) "
+              f"kkt {s.relative_kkt:.1e}; simplex {exact:.12g}; rel {rel:.1e} (FP32 phase alone "
+              f"{rel32:.1e})", flush=True)
+        assert 0 < s.stats["fp32_iterations"] < s.iterations
+        assert s.relative_kkt <= 2.5e-8
+        assert rel <= 1e-7 and rel < rel32

@@ -366,3 +366,20 @@ def test_validate_messages_equal_the_reference(case):
     got = "" if rc == 0 else lib.sankhya_last_error().decode()
     assert got == want, (name, got, want)
     assert (name == "well-formed") == (want == "")
+
+
+def test_threaded_transpose_gather_repeats_the_single_thread_run():
+    """The reference's only parallel loop is the A^T y column gather (src/la/sparse.cpp:244-259),
+    one writer per column, so a run is the same at any thread count; the oracle's all-cores
+    leg of the CPU baseline relies on that."""
+    d = cm.generated("zipf", 2000, 5000, 10, 3)[1]
+    P = orc.Problem(d)
+    runs = []
+    for t in (1, 3, 8):
+        orc.set_threads(t)
+        runs.append(orc.solve(P, tolerance=1e-6, iteration_limit=400))
+    orc.set_threads(1)
+    for r in runs[1:]:
+        assert r["iterations"] == runs[0]["iterations"]
+        assert np.array_equal(r["col_value"], runs[0]["col_value"])
+        assert np.array_equal(r["row_dual"], runs[0]["row_dual"])
