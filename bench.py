@@ -63,6 +63,8 @@ WORKLOADS = {
     # The 1,000 x 3,000 uniform-equality LP, stopped by the reference's own test at 1e-6
     # (pdhg_stop_at_request: relative residuals and absolute primal <= 1e-7).
     "uniform-1k-3k-fp64": ("uniform", 1_000, 3_000, 5, 1.5, 1, 1e-6, "reference"),
+    # The 1,000,000 x 2,000,000 Zipf LP (nnz 20M), for profile runs of its kernels.
+    "zipf-1m-2m": ("zipf", 1_000_000, 2_000_000, 10, 1.5, 3, 1e-6, "relative_kkt"),
 }
 
 
@@ -392,6 +394,9 @@ def main():
     ap.add_argument("--no-c2", action="store_true", help="skip the 1M x 2M section")
     ap.add_argument("--adjoint-lps", type=int, default=64,
                     help="blend LPs in the adjoint check (0 skips it)")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="for rocprofv3: one device handle only, one resident solve of the "
+                         "workload and the kernel timing launches, then a short JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -412,6 +417,22 @@ def main():
     log(f"[bench] generated m={M} n={N} nnz={NNZ} in {time.perf_counter()-t:.1f}s")
     opts = dict(pdhg_tolerance=tol, pdhg_stop_at_request=stop_req, pdhg_stop_rule=rule,
                 iteration_limit=args.iteration_limit, pdhg_precision=args.precision)
+    if args.profile_run:
+        # One handle for the whole process: rocprofv3's kernel trace has crashed the host process
+        # (inside the profiler, during hipGraphLaunch) at the first graph launch of a second
+        # handle created after the first was released; with one handle it traces cleanly.
+        dev = sh.Device(model, precision=args.precision)
+        t = time.perf_counter()
+        st, *_ = dev.solve(tolerance=tol, stop_rule=1 if rule == "relative_kkt" else 0,
+                           stop_at_request=stop_req, iteration_limit=args.iteration_limit)
+        ts = time.perf_counter() - t
+        p_ms, d_ms = dev.time_kernels(args.timing_reps)
+        dev.close()
+        log(f"[bench] profile run: {st.iteration} iterations in {ts:.3f}s; k_primal {p_ms*1e3:.2f} us, "
+            f"k_dual {d_ms*1e3:.2f} us")
+        print(json.dumps({"mode": "profile-run", "workload": args.workload, "iterations": int(st.iteration),
+                          "seconds": ts, "k_primal_us": p_ms * 1e3, "k_dual_us": d_ms * 1e3}), flush=True)
+        return
 
     # ---- headline: whole solve() calls, upload included ----
     for w in range(args.warmup):

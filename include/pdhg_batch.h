@@ -55,6 +55,27 @@ int pdhg_batch_upload(pdhg_batch* h, int32_t nlp, const int32_t* m, const int32_
                       const double* col_upper, const double* row_lower, const double* row_upper,
                       const double* bound_norm, const double* cost_norm);
 
+/* The same upload from each LP's UNSCALED problem, preconditioned on the device as
+ * pdhg_dev_upload_unscaled does for one LP: `ruiz_iterations` Ruiz rounds and one Pock-Chambolle
+ * round per LP (src/la/scaling.cpp:14-159), bit-identical to the host passes, then the scaled
+ * cost and bounds. a_val / at_val hold A's own values; min_cost is sense * c per LP; the norms
+ * are of the unscaled problems (src/pdhg/pdhg.cpp:94-111). */
+int pdhg_batch_upload_unscaled(pdhg_batch* h, int32_t nlp, const int32_t* m, const int32_t* n,
+                               const int64_t* nnz, const int32_t* a_rowptr, const int32_t* a_col,
+                               const double* a_val, const int32_t* at_rowptr, const int32_t* at_col,
+                               const double* at_val, const double* min_cost,
+                               const double* col_lower, const double* col_upper,
+                               const double* row_lower, const double* row_upper,
+                               const double* bound_norm, const double* cost_norm,
+                               int ruiz_iterations);
+
+/* Test hook: the batch's preconditioning results, concatenated LP after LP: Dr (sum m), Dc
+ * (sum n), the scaled values in CSC order (sum nnz) and the scaled cost and bounds; any pointer
+ * may be null. */
+int pdhg_batch_scaling(pdhg_batch* h, double* row, double* column, double* csc_values,
+                       double* cost_hat, double* col_lower_hat, double* col_upper_hat,
+                       double* row_lower_hat, double* row_upper_hat);
+
 /* Power iteration per LP (src/la/scaling.cpp:161-194): v0 holds each LP's start vector
  * (n[k] entries per LP); norms receives nlp values, each already rounded up by 1.01. */
 int pdhg_batch_spectral_norms(pdhg_batch* h, const double* v0, int iterations, double* norms);

@@ -12,6 +12,7 @@
 
 #include <stdint.h>
 
+#include "pdhg_batch.h"
 #include "pdhg_hip.h"
 
 #ifdef __cplusplus
@@ -140,6 +141,12 @@ int sankhya_pdhg_csr(const sankhya_model* model, int threads, int32_t* rowptr, i
  * and bound norm through the handle. */
 int sankhya_pdhg_device_open(const sankhya_model* model, int device, int precision,
                              pdhg_dev** out);
+
+/* Opens a batch handle holding these models exactly as the batched engine uploads them (the
+ * unscaled problems, preconditioned on the device by pdhg_batch_upload_unscaled), for the
+ * pdhg_batch_* test hooks. */
+int sankhya_pdhg_batch_open(const sankhya_model* const* models, int32_t nlp, int device,
+                            int precision, pdhg_batch** out);
 
 /* Solution::recompute_quality of the engine (the quality fields solve() reports; reference
  * src/core/model.cpp:247-441) for the point (col_value n, row_dual m, col_dual n) with the given

@@ -31,6 +31,7 @@
 #include "pdhg_kernels.hpp"
 #include "batch_layout.hpp"
 #include "plan_host.hpp"
+#include "scaling_waves.hpp"
 
 namespace sankhya_hip {
 namespace batch {
@@ -296,6 +297,67 @@ __global__ void k_batch_init(Iter<X> it) {
     init_col(it, j);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < it.m; i += stride)
     init_row(it, i);
+}
+
+// ---- preconditioning of every LP on the device (src/la/scaling.cpp:14-159) ---------------
+// The single-LP engine's passes over the concatenated batch, one wave per line (row of A or
+// column of A, i.e. row of A^T) of every LP: the line's LP comes from row_lp / col_lp, its
+// entries from the LP's own pointers, and the other side's index is offset by the LP's first
+// column (row). Results equal the host passes of each LP bit for bit (scaling_waves.hpp).
+struct BatchLines {
+  const int32_t* ptr;   // a_rowptr (side 0) or t_rowptr (side 1), per-LP local pointers
+  const int32_t* idx;   // a_col / t_col, local indices of the other side
+  double* val;          // a_val64 / t_val64
+  const LpDesc* desc;
+  const int32_t* line_lp;
+  int64_t lines;
+  int32_t side;
+};
+
+__device__ __forceinline__ void batch_line(const BatchLines& L, int64_t line, int32_t& k0,
+                                           int32_t& k1, int64_t& base, int32_t& other0) {
+  const LpDesc& d = L.desc[L.line_lp[line]];
+  const int32_t local = static_cast<int32_t>(line - (L.side ? d.col0 : d.row0));
+  const int32_t* p = L.ptr + (L.side ? d.trp0 : d.arp0);
+  base = d.nnz0;
+  k0 = p[local];
+  k1 = p[local + 1];
+  other0 = L.side ? d.row0 : d.col0;
+}
+
+// mode 0: largest |a| of each line into stat; 1: Pock-Chambolle sum of |a| in storage order.
+__global__ __launch_bounds__(kThreads) void k_bsc_stat(BatchLines L, int mode, double* stat) {
+  __shared__ double stage[kThreads * kScU];
+  const int lane = threadIdx.x & 63;
+  const int64_t line = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
+  if (line >= L.lines) return;  // the whole wave leaves together
+  int32_t k0, k1, other0;
+  int64_t base;
+  batch_line(L, line, k0, k1, base, other0);
+  const double* v = L.val + base;
+  const double r = mode == 0 ? wave_absmax(v, k0, k1, lane)
+                             : wave_abssum_ordered(v, k0, k1, lane, stage + (threadIdx.x >> 6) * 64 * kScU);
+  if (lane == 0) stat[line] = r;
+}
+
+// Every entry of the line becomes (a * r_i) * c_j: rows scale by their own factor first,
+// columns by the row's (the order of the reference's pass, src/la/scaling.cpp:14-40).
+__global__ __launch_bounds__(kThreads) void k_bsc_apply(BatchLines L, const double* line_pass,
+                                                        const double* other_pass) {
+  const int lane = threadIdx.x & 63;
+  const int64_t line = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
+  if (line >= L.lines) return;
+  int32_t k0, k1, other0;
+  int64_t base;
+  batch_line(L, line, k0, k1, base, other0);
+  wave_apply(L.idx + base, L.val + base, k0, k1, lane, line_pass[line], other_pass + other0,
+             L.side == 0 ? 1 : 0);
+}
+
+__global__ __launch_bounds__(kThreads) void k_bsc_narrow(const double* v, float* f, int64_t count) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; k < count;
+       k += static_cast<int64_t>(gridDim.x) * kThreads)
+    f[k] = static_cast<float>(v[k]);
 }
 
 // Exact FP32 -> FP64 widening of every LP's iterate pairs (the mixed-precision handoff):
@@ -872,6 +934,8 @@ struct pdhg_batch {
   std::string err;
   int32_t nlp = 0;
   int64_t sum_m = 0, sum_n = 0, sum_nnz = 0;
+  int64_t padded_total = 0;  // entries of the concatenated (per-LP even-aligned) value arrays
+  std::vector<int64_t> lp_nnz;  // nonzeros of each LP
   std::vector<LpDesc> desc_host;
   std::vector<int32_t> geo;                 // per-LP geometry
   std::array<std::vector<int32_t>, kGeos> order;  // per geometry: its LPs, largest first
@@ -1295,6 +1359,44 @@ int check(pdhg_batch* h, bool need_start) {
   return PDHG_OK;
 }
 
+// Every LP's preconditioner on the device: `ruiz` Ruiz rounds and one Pock-Chambolle round in
+// place in both copies of the values, Dr and Dc, then the scaled cost and bounds, and the FP32
+// copies of the values in mixed precision. Scratch: sy and yrest hold the row statistics and
+// factors, sx and xrest the column ones (start_state sets every iterate array afterwards).
+int batch_device_scale(pdhg_batch* h, int ruiz) {
+  const int64_t sm = h->sum_m, sn = h->sum_n;
+  const BatchLines rows{h->a_rowptr, h->a_col, h->a_val64, h->desc, h->row_lp, sm, 0};
+  const BatchLines cols{h->t_rowptr, h->t_col, h->t_val64, h->desc, h->col_lp, sn, 1};
+  double *rstat = h->sy, *rpass = h->yrest, *cstat = h->sx, *cpass = h->xrest;
+  auto waves = [](int64_t lines) { return static_cast<unsigned>((lines + kThreads / 64 - 1) / (kThreads / 64)); };
+  auto lanes = [](int64_t count) {
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((count + kThreads - 1) / kThreads, 8192)));
+  };
+  if (sm > 0) k_sc_fill<<<lanes(sm), kThreads, 0, h->stream>>>(h->dr, static_cast<int32_t>(sm), 1.0);
+  if (sn > 0) k_sc_fill<<<lanes(sn), kThreads, 0, h->stream>>>(h->dc, static_cast<int32_t>(sn), 1.0);
+  auto round = [&](int mode) {
+    if (sm > 0) k_bsc_stat<<<waves(sm), kThreads, 0, h->stream>>>(rows, mode, rstat);
+    if (sn > 0) k_bsc_stat<<<waves(sn), kThreads, 0, h->stream>>>(cols, mode, cstat);
+    if (sm > 0) k_sc_factor<<<lanes(sm), kThreads, 0, h->stream>>>(rstat, rpass, h->dr, static_cast<int32_t>(sm));
+    if (sn > 0) k_sc_factor<<<lanes(sn), kThreads, 0, h->stream>>>(cstat, cpass, h->dc, static_cast<int32_t>(sn));
+    if (sm > 0) k_bsc_apply<<<waves(sm), kThreads, 0, h->stream>>>(rows, rpass, cpass);
+    if (sn > 0) k_bsc_apply<<<waves(sn), kThreads, 0, h->stream>>>(cols, cpass, rpass);
+  };
+  for (int pass = 0; pass < ruiz; ++pass) round(0);
+  round(1);
+  const int64_t most = std::max<int64_t>(std::max(sm, sn), 1);
+  k_sc_vectors<<<lanes(most), kThreads, 0, h->stream>>>(
+      static_cast<int32_t>(sn), static_cast<int32_t>(sm), h->dc, h->dr, h->cost, h->cl, h->cu, h->rl,
+      h->ru, h->chat, h->lhat, h->uhat, h->rlhat, h->ruhat);
+  if (h->precision == PDHG_PRECISION_MIXED && h->padded_total > 0) {
+    k_bsc_narrow<<<lanes(h->padded_total), kThreads, 0, h->stream>>>(h->a_val64, h->a_val32, h->padded_total);
+    k_bsc_narrow<<<lanes(h->padded_total), kThreads, 0, h->stream>>>(h->t_val64, h->t_val32, h->padded_total);
+  }
+  BHIP_TRY(h, hipGetLastError());
+  BHIP_TRY(h, hipStreamSynchronize(h->stream));
+  return PDHG_OK;
+}
+
 // Control blocks at the start point (src/pdhg/pdhg.cpp:137-192), iterates initialised, and the
 // delta form's anchors.
 int start_state(pdhg_batch* h, const pdhg_params* p, const double* norms) {
@@ -1408,6 +1510,7 @@ int pdhg_batch_upload(pdhg_batch* h, int32_t nlp, const int32_t* m, const int32_
   BHIP_TRY(h, hipSetDevice(h->device));
   h->nlp = nlp;
   h->desc_host = L.desc;
+  h->lp_nnz.assign(nnz, nnz + nlp);
   h->geo.assign(static_cast<size_t>(nlp), kDefaultGeo);
   h->sum_m = L.sum_m;
   h->sum_n = L.sum_n;
@@ -1415,6 +1518,7 @@ int pdhg_batch_upload(pdhg_batch* h, int32_t nlp, const int32_t* m, const int32_
   const int64_t sm = L.sum_m, sn = L.sum_n;
   {
     const size_t total = L.a_col.size();
+    h->padded_total = static_cast<int64_t>(total);
     int64_t arp = 0, trp = 0;
     for (int32_t k = 0; k < nlp; ++k) {
       arp += m[k] + 1;
@@ -1512,6 +1616,59 @@ int pdhg_batch_upload(pdhg_batch* h, int32_t nlp, const int32_t* m, const int32_
   BHIP_TRY(h, hipStreamSynchronize(h->stream));
   if ((rc = set_bins(h))) return rc;
   h->uploaded = true;
+  return PDHG_OK;
+}
+
+int pdhg_batch_upload_unscaled(pdhg_batch* h, int32_t nlp, const int32_t* m, const int32_t* n,
+                               const int64_t* nnz, const int32_t* a_rowptr, const int32_t* a_col,
+                               const double* a_val, const int32_t* at_rowptr, const int32_t* at_col,
+                               const double* at_val, const double* min_cost,
+                               const double* col_lower, const double* col_upper,
+                               const double* row_lower, const double* row_upper,
+                               const double* bound_norm, const double* cost_norm,
+                               int ruiz_iterations) {
+  if (!h) return PDHG_E_ARGUMENT;
+  if (ruiz_iterations < 0) return bfail(h, PDHG_E_ARGUMENT, "negative Ruiz iteration count");
+  // The unscaled vectors stand in for the scaled ones until the device passes overwrite them.
+  int rc = pdhg_batch_upload(h, nlp, m, n, nnz, a_rowptr, a_col, a_val, at_rowptr, at_col, at_val,
+                             min_cost, col_lower, col_upper, row_lower, row_upper, nullptr,
+                             nullptr, min_cost, col_lower, col_upper, row_lower, row_upper,
+                             bound_norm, cost_norm);
+  if (rc) return rc;
+  return batch_device_scale(h, ruiz_iterations);
+}
+
+int pdhg_batch_scaling(pdhg_batch* h, double* row, double* column, double* csc_values,
+                       double* cost_hat, double* col_lower_hat, double* col_upper_hat,
+                       double* row_lower_hat, double* row_upper_hat) {
+  int rc = check(h, false);
+  if (rc) return rc;
+  BHIP_TRY(h, hipSetDevice(h->device));
+  const size_t un = static_cast<size_t>(h->sum_n), um = static_cast<size_t>(h->sum_m);
+  struct Item {
+    double* dst;
+    const double* src;
+    size_t count;
+  } items[] = {{row, h->dr, um},          {column, h->dc, un},          {cost_hat, h->chat, un},
+               {col_lower_hat, h->lhat, un}, {col_upper_hat, h->uhat, un}, {row_lower_hat, h->rlhat, um},
+               {row_upper_hat, h->ruhat, um}};
+  for (const Item& it : items)
+    if (it.dst && it.count)
+      BHIP_TRY(h, hipMemcpyAsync(it.dst, it.src, sizeof(double) * it.count, hipMemcpyDeviceToHost, h->stream));
+  if (csc_values && h->sum_nnz > 0) {
+    std::vector<double> all(static_cast<size_t>(h->padded_total));
+    BHIP_TRY(h, hipMemcpyAsync(all.data(), h->t_val64, sizeof(double) * all.size(), hipMemcpyDeviceToHost, h->stream));
+    BHIP_TRY(h, hipStreamSynchronize(h->stream));
+    // Each LP's entries open its (even-aligned) block of the concatenated array.
+    int64_t out = 0;
+    for (int32_t k = 0; k < h->nlp; ++k) {
+      const int64_t count = h->lp_nnz[static_cast<size_t>(k)];
+      std::memcpy(csc_values + out, all.data() + h->desc_host[k].nnz0,
+                  sizeof(double) * static_cast<size_t>(count));
+      out += count;
+    }
+  }
+  BHIP_TRY(h, hipStreamSynchronize(h->stream));
   return PDHG_OK;
 }
 

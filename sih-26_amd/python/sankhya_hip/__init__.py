@@ -132,6 +132,9 @@ def load_library() -> C.CDLL:
     lib.pdhg_dev_finish.argtypes = [vp, _f64p, _f64p, _f64p, _f64p]
     lib.sankhya_pdhg_csr.argtypes = [vp, C.c_int, _i32p, _i32p, _f64p]
     lib.sankhya_pdhg_quality.argtypes = [vp, _f64p, _f64p, _f64p, C.c_double, _f64p, _f64p]
+    lib.sankhya_pdhg_batch_open.argtypes = [C.POINTER(vp), C.c_int32, C.c_int, C.c_int, C.POINTER(vp)]
+    lib.pdhg_batch_scaling.argtypes = [vp] + [_f64p] * 8
+    lib.pdhg_batch_destroy.argtypes = [vp]
     lib.sankhya_pdhg_power_start.argtypes = [vp, C.c_int64, _f64p]
     lib.pdhg_dev_precondition.argtypes = [vp, C.POINTER(C.c_double)]
     lib.pdhg_dev_scaling.argtypes = [vp] + [_f64p] * 8
@@ -491,6 +494,34 @@ def host_scaling(model: Model, ruiz: int = 10, threads: int = 1) -> Dict[str, np
                   ("col_upper", n), ("row_lower", m), ("row_upper", m)):
         out[k] = out[k][:ln]
     return out
+
+
+def batch_device_scaling(models, precision: str = "fp64", device: int = 0) -> Dict[str, np.ndarray]:
+    """The batched engine's device preconditioning of `models` (pdhg_batch_upload_unscaled),
+    read back with pdhg_batch_scaling: row (Dr), column (Dc), val (CSC order), cost, col_lower,
+    col_upper, row_lower, row_upper, each concatenated LP after LP."""
+    lib = load_library()
+    arr = (C.c_void_p * len(models))(*[m._h for m in models])
+    h = C.c_void_p()
+    rc = lib.sankhya_pdhg_batch_open(arr, len(models), device, PRECISION[precision], C.byref(h))
+    if rc != 0:
+        raise RuntimeError(f"batch open failed with code {rc}")
+    sm = sum(m.dims[0] for m in models)
+    sn = sum(m.dims[1] for m in models)
+    sz = sum(m.dims[2] for m in models)
+    out = {k: np.zeros(max(c, 1)) for k, c in (("row", sm), ("column", sn), ("val", sz), ("cost", sn),
+                                              ("col_lower", sn), ("col_upper", sn), ("row_lower", sm),
+                                              ("row_upper", sm))}
+    try:
+        rc = lib.pdhg_batch_scaling(h, out["row"], out["column"], out["val"], out["cost"],
+                                    out["col_lower"], out["col_upper"], out["row_lower"], out["row_upper"])
+        if rc != 0:
+            raise RuntimeError(f"pdhg_batch_scaling failed ({rc})")
+    finally:
+        lib.pdhg_batch_destroy(h)
+    cut = {"row": sm, "column": sn, "val": sz, "cost": sn, "col_lower": sn, "col_upper": sn,
+           "row_lower": sm, "row_upper": sm}
+    return {k: v[: cut[k]] for k, v in out.items()}
 
 
 class Device:

@@ -415,6 +415,33 @@ int sankhya_pdhg_device_open(const sankhya_model* h, int device, int precision, 
   return rc;
 }
 
+int sankhya_pdhg_batch_open(const sankhya_model* const* models, int32_t nlp, int device,
+                            int precision, pdhg_batch** out) {
+  if (!models || nlp <= 0 || !out) return PDHG_E_ARGUMENT;
+  *out = nullptr;
+  std::vector<const Model*> ms;
+  for (int32_t k = 0; k < nlp; ++k) {
+    if (!models[k]) return PDHG_E_ARGUMENT;
+    ms.push_back(&models[k]->model);
+  }
+  std::vector<hip_engine::Prepared> prep;
+  std::vector<std::vector<double>> v0;
+  hip_engine::prepare_all(ms, 0, &prep, &v0, /*scale_on_host=*/false);
+  const hip_engine::BatchInput in = hip_engine::make_batch_input(ms, prep, v0, /*host_scaled=*/false);
+  int rc = pdhg_batch_create(device, precision, out);
+  if (rc) return rc;
+  rc = pdhg_batch_upload_unscaled(*out, nlp, in.m.data(), in.n.data(), in.nnz.data(), in.a_rp.data(),
+                                  in.a_ci.data(), in.a_v.data(), in.t_rp.data(), in.t_ci.data(),
+                                  in.t_v.data(), in.cost.data(), in.cl.data(), in.cu.data(),
+                                  in.rl.data(), in.ru.data(), in.bound_norm.data(),
+                                  in.cost_norm.data(), hip_engine::kRuizIterations);
+  if (rc) {
+    pdhg_batch_destroy(*out);
+    *out = nullptr;
+  }
+  return rc;
+}
+
 int sankhya_pdhg_quality(const sankhya_model* h, const double* col_value, const double* row_dual,
                          const double* col_dual, double dual_bound, double* row_activity,
                          double* out) {

@@ -240,7 +240,8 @@ int engine_device(const Options& options, Logger& logger) {
 }
 
 void prepare_all(const std::vector<const Model*>& models, std::int64_t random_seed,
-                 std::vector<Prepared>* prep, std::vector<std::vector<double>>* v0) {
+                 std::vector<Prepared>* prep, std::vector<std::vector<double>>* v0,
+                 bool scale_on_host) {
   const std::size_t count = models.size();
   prep->assign(count, Prepared{});
   v0->assign(count, std::vector<double>{});
@@ -249,7 +250,7 @@ void prepare_all(const std::vector<const Model*>& models, std::int64_t random_se
   std::atomic<std::size_t> next{0};
   auto work = [&] {
     for (std::size_t k; (k = next.fetch_add(1)) < count;) {
-      (*prep)[k] = prepare(*models[k]);
+      (*prep)[k] = scale_on_host ? prepare(*models[k]) : prepare_unscaled(*models[k]);
       (*v0)[k] = power_start(*models[k], random_seed);
     }
   };
@@ -261,7 +262,7 @@ void prepare_all(const std::vector<const Model*>& models, std::int64_t random_se
 
 BatchInput make_batch_input(const std::vector<const Model*>& models,
                             const std::vector<Prepared>& prep,
-                            const std::vector<std::vector<double>>& v0) {
+                            const std::vector<std::vector<double>>& v0, bool host_scaled) {
   BatchInput b;
   for (std::size_t q = 0; q < models.size(); ++q) {
     const Model& md = *models[q];
@@ -274,17 +275,20 @@ BatchInput make_batch_input(const std::vector<const Model*>& models,
     b.a_v.insert(b.a_v.end(), p.a_v.begin(), p.a_v.end());
     b.t_rp.insert(b.t_rp.end(), md.matrix.column_starts().begin(), md.matrix.column_starts().end());
     b.t_ci.insert(b.t_ci.end(), md.matrix.row_indices().begin(), md.matrix.row_indices().end());
-    b.t_v.insert(b.t_v.end(), p.scaling.values.begin(), p.scaling.values.end());
+    b.t_v.insert(b.t_v.end(), host_scaled ? p.scaling.values.begin() : md.matrix.values().begin(),
+                 host_scaled ? p.scaling.values.end() : md.matrix.values().end());
     auto add = [](std::vector<double>& dst, const std::vector<double>& src) {
       dst.insert(dst.end(), src.begin(), src.end());
     };
-    add(b.chat, p.scaling.cost);
-    add(b.lhat, p.scaling.col_lower);
-    add(b.uhat, p.scaling.col_upper);
-    add(b.rlhat, p.scaling.row_lower);
-    add(b.ruhat, p.scaling.row_upper);
-    add(b.dc, p.scaling.column);
-    add(b.dr, p.scaling.row);
+    if (host_scaled) {
+      add(b.chat, p.scaling.cost);
+      add(b.lhat, p.scaling.col_lower);
+      add(b.uhat, p.scaling.col_upper);
+      add(b.rlhat, p.scaling.row_lower);
+      add(b.ruhat, p.scaling.row_upper);
+      add(b.dc, p.scaling.column);
+      add(b.dr, p.scaling.row);
+    }
     add(b.cost, p.cost);
     add(b.cl, md.col_lower);
     add(b.cu, md.col_upper);

@@ -89,11 +89,15 @@ int host_threads();
 
 int engine_device(const Options& options, Logger& logger);
 
-// prepare() and power_start() for every model, on all host cores (one model per task).
+// prepare() (scale_on_host) or prepare_unscaled() and power_start() for every model, on all
+// host cores (one model per task).
 void prepare_all(const std::vector<const Model*>& models, std::int64_t random_seed,
-                 std::vector<Prepared>* prep, std::vector<std::vector<double>>* v0);
+                 std::vector<Prepared>* prep, std::vector<std::vector<double>>* v0,
+                 bool scale_on_host = true);
 
-// A batch's arrays, concatenated LP after LP in the layout include/pdhg_batch.h takes.
+// A batch's arrays, concatenated LP after LP in the layout include/pdhg_batch.h takes. From
+// prepare_unscaled() results the values are A's own and the scaled vectors and Dr/Dc are left
+// empty (pdhg_batch_upload_unscaled).
 struct BatchInput {
   std::vector<int32_t> m, n;
   std::vector<int64_t> nnz;
@@ -103,6 +107,6 @@ struct BatchInput {
 };
 BatchInput make_batch_input(const std::vector<const Model*>& models,
                             const std::vector<Prepared>& prep,
-                            const std::vector<std::vector<double>>& v0);
+                            const std::vector<std::vector<double>>& v0, bool host_scaled = true);
 
 }  // namespace sankhya::hip_engine

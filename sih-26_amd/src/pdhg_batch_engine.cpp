@@ -84,8 +84,10 @@ std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models
   for (const std::size_t k : live) live_models.push_back(models[k]);
   std::vector<Prepared> prep;
   std::vector<std::vector<double>> v0;
-  prepare_all(live_models, options.get_int("random_seed"), &prep, &v0);
-  const BatchInput in = make_batch_input(live_models, prep, v0);
+  // Norms and the CSR of A on the host, one LP per task; the preconditioner runs on the device
+  // for every LP at once (pdhg_batch_upload_unscaled).
+  prepare_all(live_models, options.get_int("random_seed"), &prep, &v0, /*scale_on_host=*/false);
+  const BatchInput in = make_batch_input(live_models, prep, v0, /*host_scaled=*/false);
   const std::vector<int32_t>& m = in.m;
   const std::vector<int32_t>& n = in.n;
   const double setup_seconds = timer.elapsed_seconds();
@@ -98,12 +100,11 @@ std::unique_ptr<BatchSession> open_batch(const std::vector<const Model*>& models
     return fail_all(std::string(step) + " failed (code " + std::to_string(code) + "): " +
                     pdhg_batch_last_error(dev_h));
   };
-  rc = pdhg_batch_upload(dev_h, static_cast<int32_t>(nb), in.m.data(), in.n.data(), in.nnz.data(),
-                         in.a_rp.data(), in.a_ci.data(), in.a_v.data(), in.t_rp.data(),
-                         in.t_ci.data(), in.t_v.data(), in.chat.data(), in.lhat.data(),
-                         in.uhat.data(), in.rlhat.data(), in.ruhat.data(), in.dc.data(),
-                         in.dr.data(), in.cost.data(), in.cl.data(), in.cu.data(), in.rl.data(),
-                         in.ru.data(), in.bound_norm.data(), in.cost_norm.data());
+  rc = pdhg_batch_upload_unscaled(dev_h, static_cast<int32_t>(nb), in.m.data(), in.n.data(),
+                                  in.nnz.data(), in.a_rp.data(), in.a_ci.data(), in.a_v.data(),
+                                  in.t_rp.data(), in.t_ci.data(), in.t_v.data(), in.cost.data(),
+                                  in.cl.data(), in.cu.data(), in.rl.data(), in.ru.data(),
+                                  in.bound_norm.data(), in.cost_norm.data(), kRuizIterations);
   if (rc != PDHG_OK) return dev_error(rc, "upload");
   std::vector<double> norms(nb, 1.0);
   rc = pdhg_batch_spectral_norms(dev_h, in.v0.data(), kPowerIterations, norms.data());

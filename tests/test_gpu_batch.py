@@ -206,3 +206,71 @@ def test_mixed_batch_polished_toward_1e8_matches_the_exact_optimum():
         assert 0 < s.stats["fp32_iterations"] < s.iterations
         assert s.relative_kkt <= 2.5e-8
         assert rel <= 1e-7 and rel < rel32
+
+
+def test_full_512_lp_batched_workload_checked_by_the_oracle():
+    """The batched workload at its full size (512 LPs, m log-uniform in [200, 20000], n = 2.5 m,
+    6 nonzeros per column, 33M nonzeros): one mixed-precision batch to relative KKT 1e-6. Every
+    LP's reported point is evaluated again by the oracle's evaluate() on the CPU (the reference's
+    src/pdhg/pdhg_evaluate.cpp:19-139), which must give relative KKT <= 1e-6 too; a stratified
+    sample (every 32nd LP, smallest to largest) is solved again one by one by the plain FP64
+    engine. Two points that each stop at relative KKT 1e-6 need not share an objective to 1e-5:
+    for a point (x, y) with primal objective p, dual objective d and absolute residual norms
+    r_p, r_d, weak duality puts the optimum within |p - d| + |y| r_p + |x| r_d of p (with |y|,
+    |x| standing in for the optimal norms, doubled for that), so the two objectives must agree
+    within the sum of their two envelopes, from the oracle's evaluate() of each point."""
+    import sankhya_hip.workloads as wl
+    models = wl.batched(512)
+    opts = dict(pdhg_tolerance=1e-6, pdhg_stop_rule="relative_kkt", iteration_limit=1000000)
+    t = time.time()
+    sols = sh.solve_batch(models, pdhg_precision="mixed", **opts)
+    print(f"512 LPs in {time.time()-t:.1f}s; iterations median "
+          f"{int(np.median([s.iterations for s in sols]))}, max {max(s.iterations for s in sols)}", flush=True)
+    worst = 0.0
+    for md, s in zip(models, sols):
+        assert s.reached(1e-6), (md.dims, s.status, s.message)
+        P = orc.Problem(md.arrays())
+        _, _, r = orc.evaluate(P, s.col_value, -s.row_dual)
+        kkt = max(r[0], r[1], r[2])
+        worst = max(worst, kkt)
+        assert kkt <= 1e-6 * (1 + 1e-9), (md.dims, kkt, s.relative_kkt)
+    order = sorted(range(len(models)), key=lambda k: models[k].dims[0])
+    sample = order[::32]
+    t = time.time()
+    def envelope(P, s):
+        _, _, e = orc.evaluate(P, s.col_value, -s.row_dual)
+        return (abs(e[3] - e[4]) + 2 * np.linalg.norm(s.row_dual) * e[5]
+                + 2 * np.linalg.norm(s.col_value) * e[6])
+
+    rel_worst, used_worst = 0.0, 0.0
+    for k in sample:
+        r = sh.solve(models[k], **opts)
+        assert r.reached(1e-6), (models[k].dims, r.status, r.message)
+        P = orc.Problem(models[k].arrays())
+        diff = abs(r.objective - sols[k].objective)
+        allowed = envelope(P, r) + envelope(P, sols[k])
+        rel_worst = max(rel_worst, diff / max(1.0, abs(r.objective)))
+        used_worst = max(used_worst, diff / allowed)
+        assert diff <= allowed, (models[k].dims, diff, allowed)
+    print(f"oracle relative KKT at most {worst:.2e}; {len(sample)} FP64 single solves in "
+          f"{time.time()-t:.1f}s, objective differences at most {rel_worst:.1e} relative and "
+          f"{used_worst:.2f} of the weak-duality envelope", flush=True)
+
+
+def test_batch_device_preconditioning_is_bit_identical_to_the_host():
+    """The batched engine preconditions every LP on the device (pdhg_batch_upload_unscaled):
+    Dr, Dc, the scaled values and the scaled cost and bounds of each LP equal the host
+    preconditioner's (sankhya_pdhg_scaling, itself pinned to the reference's build_scaling in
+    tests/test_oracle.py) bit for bit, for LPs of uneven sizes and shapes in one batch: no rows,
+    empty rows and columns, dense columns, Netlib LPs, the batched workload."""
+    import sankhya_hip.workloads as wl
+    models = [cm.model_of(d) for _, d, _, _ in cm.DEGENERATE] + \
+             [cm.model_of(cm.netlib(nm)) for nm in ("afiro", "blend", "israel")] + \
+             [cm.transposed_zipf(2000, 5000, 3)[0], cm.c0(1)[0]] + wl.batched(8)
+    for precision in ("fp64", "mixed"):
+        got = sh.batch_device_scaling(models, precision=precision)
+        host = [sh.host_scaling(md, 10, 1) for md in models]
+        for k in got:
+            want = np.concatenate([h[k] for h in host])
+            assert np.array_equal(got[k], want), (precision, k, np.flatnonzero(got[k] != want)[:5])
+    print(f"{len(models)} LPs, {sum(md.dims[2] for md in models)} nonzeros: device = host bit for bit", flush=True)
