@@ -72,15 +72,40 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(m, n, nnz):
-    """Algorithmic HBM bytes per launch of the two iteration kernels (FP64 values, int32
-    indices). k_dual streams the CSR of Ahat once for both right-hand sides (ext and dx come
-    from one gather of the (x, x') pair) and updates y; k_primal streams the CSR of Ahat^T and
-    updates x. Counted once per unique element: the matrix (12 nnz + 4 (rows+1)), the
-    gathered vector (16 n for the x pairs / 8 m for y), and the epilogue's per-row vectors."""
-    k_dual = 12 * nnz + 4 * (m + 1) + 16 * n + m * (8 + 16 + 8)  # y, row bounds, y' write
-    k_primal = 12 * nnz + 4 * (n + 1) + 8 * m + n * (8 + 24 + 8)  # x, cost+bounds, x' write
+def kernel_bytes(m, n, nnz, precision="fp64"):
+    """Algorithmic HBM bytes per launch of the two iteration kernels (int32 indices). k_dual
+    streams the CSR of Ahat once for both right-hand sides (ext and dx come from one gather of
+    the (x, x') pair) and updates y; k_primal streams the CSR of Ahat^T and updates x. Counted
+    once per unique element: the matrix (value + index per nonzero, 4 (rows+1) of row pointers),
+    the gathered vector, and the epilogue's per-row vectors.
+      fp64: values 8 B; x pairs 16 B, y 8 B; per row y, row bounds, y' (32 B); per column x,
+            cost and bounds, x' (40 B).
+      mixed (the FP32 phase, FP32 deltas around an FP64 restart point): values 4 B; x pairs 8 B,
+            y 4 B; per row the FP32 y and y' (8 B), the bounds (16 B) and the FP64 restart point
+            and its product yrest, Ahat xrest (16 B), 40 B; per column 48 B likewise."""
+    if precision == "mixed":
+        k_dual = 8 * nnz + 4 * (m + 1) + 8 * n + m * (4 + 16 + 16 + 4)
+        k_primal = 8 * nnz + 4 * (n + 1) + 4 * m + n * (4 + 24 + 16 + 4)
+    else:
+        k_dual = 12 * nnz + 4 * (m + 1) + 16 * n + m * (8 + 16 + 8)  # y, row bounds, y' write
+        k_primal = 12 * nnz + 4 * (n + 1) + 8 * m + n * (8 + 24 + 8)  # x, cost+bounds, x' write
     return k_primal, k_dual
+
+
+def kernel_roofline(sh, model, precision, spectral_norm, reps, tol=1e-6, stop_rule=1):
+    """Both iteration kernels launched standalone on the engine's stream (pdhg_dev_time_kernels:
+    HIP events around `reps` launches each) on a handle that has run one graph chunk, with
+    their algorithmic bytes (kernel_bytes) and the achieved rate."""
+    m, n, nnz = model.dims
+    dev = sh.Device(model, precision=precision)
+    dev.start(tolerance=tol, spectral_norm=spectral_norm, stop_rule=stop_rule)
+    dev.run(1)
+    p_ms, d_ms = dev.time_kernels(reps)
+    dev.close()
+    bp, bd = kernel_bytes(m, n, nnz, precision)
+    kp = {"name": "k_primal", "ms": p_ms, "bytes": bp, "gbs": bp / (p_ms * 1e6)}
+    kd = {"name": "k_dual", "ms": d_ms, "bytes": bd, "gbs": bd / (d_ms * 1e6)}
+    return kp, kd
 
 
 def traffic_bytes(kernel, args):
@@ -253,7 +278,7 @@ def reference_rule_check(sh, seeds=(1, 2, 3, 4), cpu=True):
     return out
 
 
-def c2_check(sh, iteration_cap=100_000):
+def c2_check(sh, iteration_cap=100_000, reps=20):
     """The 1,000,000 x 2,000,000 Zipf LP (nnz 20M, seed 3): mixed precision, FP32 phase to
     relative KKT 1e-6, then the FP64 polish toward 1e-8 under an iteration cap; and a plain FP64
     solve to 1e-6 whose objective the mixed result is compared with."""
@@ -273,6 +298,17 @@ def c2_check(sh, iteration_cap=100_000):
                                pdhg_precision="mixed"))
     tc = time.perf_counter() - t
     rel = lambda s: abs(s.objective - a.objective) / max(1.0, abs(a.objective))
+    # Both precisions' iteration kernels on this LP: FP32 halves the matrix values and the
+    # gathered pairs, but the FP32-delta epilogue reads the FP64 restart point as well, and a
+    # random 8-byte gather costs the same cache line as a 16-byte one.
+    kernels = {}
+    for prec in ("fp64", "mixed"):
+        kp, kd = kernel_roofline(sh, model, prec, a.stats["spectral_norm"], reps)
+        kernels[prec] = {k["name"]: {"us": k["ms"] * 1e3, "bytes_per_launch": k["bytes"],
+                                     "gbs": k["gbs"], "frac": k["gbs"] / HBM_PEAK_GBS}
+                         for k in (kp, kd)}
+        log(f"[bench] c2 {prec}: k_primal {kp['ms']*1e3:.1f} us ({kp['gbs']:.0f} GB/s), "
+            f"k_dual {kd['ms']*1e3:.1f} us ({kd['gbs']:.0f} GB/s)")
     log(f"[bench] c2: fp64 1e-6 it={a.iterations} {ta:.2f}s; mixed 1e-6 it={b.iterations} {tb:.2f}s "
         f"(rel {rel(b):.1e}); mixed+polish it={c.iterations} (fp32 {c.stats['fp32_iterations']}) "
         f"kkt {c.relative_kkt:.2e} {tc:.2f}s")
@@ -282,6 +318,7 @@ def c2_check(sh, iteration_cap=100_000):
             "mixed_1e6": {"seconds": tb, "iterations": b.iterations, "relative_kkt": b.relative_kkt,
                           "objective_rel_error_vs_fp64": rel(b), "status": b.status,
                           "speedup_vs_fp64": ta / tb if tb > 0 else None},
+            "kernels": kernels,
             "mixed_polished_toward_1e8": {
                 "seconds": tc, "iterations": c.iterations, "iteration_cap": iteration_cap,
                 "fp32_iterations": c.stats["fp32_iterations"], "relative_kkt": c.relative_kkt,
@@ -482,18 +519,9 @@ def main():
 
     # ---- roofline of the dominant kernel: standalone launches on the engine's stream, timed
     # with HIP events, on the same uploaded LP ----
-    dev = sh.Device(model, precision=args.precision)
-    dev.start(tolerance=tol, spectral_norm=last.stats["spectral_norm"],
-              stop_rule=1 if rule == "relative_kkt" else 0)
-    dev.run(1)
-    p_ms, d_ms = dev.time_kernels(args.timing_reps)
-    dev.close()
-    bp, bd = kernel_bytes(M, N, NNZ)
-    if args.precision == "mixed":
-        bp -= 4 * NNZ + 4 * N + 4 * M
-        bd -= 4 * NNZ + 8 * N + 4 * M
-    kp = {"name": "k_primal", "ms": p_ms, "bytes": bp, "gbs": bp / (p_ms * 1e6)}
-    kd = {"name": "k_dual", "ms": d_ms, "bytes": bd, "gbs": bd / (d_ms * 1e6)}
+    kp, kd = kernel_roofline(sh, model, args.precision, last.stats["spectral_norm"],
+                             args.timing_reps, tol, 1 if rule == "relative_kkt" else 0)
+    p_ms, d_ms = kp["ms"], kd["ms"]
     dom = kd if d_ms >= p_ms else kp
     log(f"[bench] k_primal {p_ms*1e3:.1f} us {kp['gbs']:.0f} GB/s; k_dual {d_ms*1e3:.1f} us {kd['gbs']:.0f} GB/s")
 
@@ -514,7 +542,7 @@ def main():
 
     c2 = None
     if not args.no_c2:
-        c2 = c2_check(sh)
+        c2 = c2_check(sh, reps=args.timing_reps)
 
     batched = None
     if args.batched_lps > 0:

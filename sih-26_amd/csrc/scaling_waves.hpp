@@ -62,35 +62,53 @@ __device__ __forceinline__ void wave_apply(const int32_t* idx, double* val, int3
   }
 }
 
-// Sum of |a| over [k0, k1) in storage order, one wave: the wave stages a step of entries in
-// its LDS slice with coalesced loads, then lane 0 adds them one by one in order, so the result
-// is the sequential sum bit for bit. Only lane 0's result is meaningful.
+// One lane's double as a wave-uniform value (two v_readlane_b32): lane `l` must be uniform.
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b), l));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), l));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// Sum of |a| over [k0, k1) in storage order, one wave, as one sequential chain of adds, so the
+// result is the sequential sum bit for bit. Each step's 64 kScU entries arrive with coalesced
+// loads; the loads of the next step are issued before the current step's entries are added,
+// so only the chain of adds (entry e is lane e % 64 of register e / 64, read with v_readlane)
+// is on the critical path, not a memory round trip per step. The chain walks blocks of 8 lanes
+// up to the last valid entry; entries past k1 in the last block are +0.0 and leave the sum (a
+// sum of magnitudes, never -0.0) unchanged. Every lane returns the sum.
 __device__ __forceinline__ double wave_abssum_ordered(const double* val, int32_t k0, int32_t k1,
-                                                      int lane, double* lds /* 64 kScU */) {
+                                                      int lane) {
   double r = 0.0;
+  if (k1 <= k0) return r;
+  // Loads are never conditional (a lane past the end reads entry k1 - 1 of this line and
+  // multiplies it by 0.0, exact for the finite entries of a matrix), so all of a step's loads
+  // stay in flight together instead of each waiting behind a branch.
+  const int32_t last = k1 - 1;
+  double v[kScU];
+#pragma unroll
+  for (int u = 0; u < kScU; ++u) {
+    const int32_t k = k0 + u * 64 + lane;
+    v[u] = fabs(val[k < last ? k : last]) * (k < k1 ? 1.0 : 0.0);
+  }
   for (int32_t base = k0; base < k1; base += 64 * kScU) {
-    double v[kScU];
+    const int32_t next = base + 64 * kScU;
+    double nv[kScU];
 #pragma unroll
     for (int u = 0; u < kScU; ++u) {
-      const int32_t k = base + u * 64 + lane;
-      v[u] = k < k1 ? fabs(val[k]) : 0.0;
+      const int32_t k = next + u * 64 + lane;
+      nv[u] = fabs(val[k < last ? k : last]) * (k < k1 ? 1.0 : 0.0);
     }
+    const int32_t count = k1 - base < 64 * kScU ? k1 - base : 64 * kScU;
 #pragma unroll
-    for (int u = 0; u < kScU; ++u) lds[u * 64 + lane] = v[u];
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0) {
-      const int32_t count = k1 - base < 64 * kScU ? k1 - base : 64 * kScU;
-      int32_t e = 0;
-      for (; e + 8 <= count; e += 8) {  // eight reads in flight, then eight adds in order
-        double t[8];
+    for (int u = 0; u < kScU; ++u) {
+      for (int lb = 0; lb < 64 && u * 64 + lb < count; lb += 8) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) t[q] = lds[e + q];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) r += t[q];
+        for (int q = 0; q < 8; ++q) r += readlane_f64(v[u], lb + q);
       }
-      for (; e < count; ++e) r += lds[e];
     }
-    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < kScU; ++u) v[u] = nv[u];
   }
   return r;
 }
