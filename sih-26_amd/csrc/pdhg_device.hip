@@ -688,8 +688,9 @@ int upload(pdhg_dev* h, T** dst, const T* src, size_t count) {
 int make_plan(pdhg_dev* h, DevPlan* P, int32_t rows, int32_t ncols, int64_t nnz,
               const int32_t* rowptr, const int32_t* col, const double* val) {
   std::vector<int4> logical;
-  const HostPlan hp = build_plan(rows, ncols, rowptr, col, &logical);
-  if (!plan_ok(hp, logical, rows, rowptr))
+  const PlanGeo geo = single_geo(ncols, h->precision == PDHG_PRECISION_MIXED ? 8 : 16);
+  const HostPlan hp = build_plan(rows, ncols, rowptr, col, &logical, geo);
+  if (!plan_ok(hp, logical, rows, rowptr, geo))
     return fail(h, PDHG_E_ARGUMENT, "internal: work plan does not cover the matrix");
   P->rows = rows;
   P->nitems = static_cast<int32_t>(hp.items.size());
@@ -1026,7 +1027,7 @@ int64_t pdhg_dev_estimate_bytes(int32_t m, int32_t n, int64_t nnz, const int32_t
     const int32_t* rp = side == 0 ? a_rowptr : at_rowptr;
     const int32_t* ci = side == 0 ? a_colidx : at_colidx;
     std::vector<int4> logical;
-    const HostPlan hp = build_plan(rows, cols, rp, ci, &logical);
+    const HostPlan hp = build_plan(rows, cols, rp, ci, &logical, single_geo(cols, mx ? 8 : 16));
     const int64_t ni = static_cast<int64_t>(hp.items.size()), nl = static_cast<int64_t>(hp.lr_nseg.size());
     total += bytes(rows + 1, 4) + bytes(nnz, 4) + bytes(nnz, 8) + (mx ? bytes(nnz, 4) : 0);
     total += bytes(ni, 16) + bytes(nl, 16) + bytes(static_cast<int64_t>(hp.groups.size()), 16);

@@ -36,6 +36,20 @@ struct PlanGeo {
   int tile = kTile, seg = kSeg, max_rows = kMaxRows, tail = kTail, bands = 8, threads = kThreads;
 };
 
+// Column bands of a single-LP plan: 8 (one per XCD) times the smallest power of two that keeps
+// one band of the gathered vector of (x, x') or (y, y') pairs within kBandBytes, half of an
+// XCD's 4 MB L2 (the other half is left to the streamed matrix). On the Zipf LPs: 200k x 500k
+// in FP64 (8 MB of x pairs) runs k_dual fastest with 8 bands; 1M x 2M in FP64 (32 MB) with 16,
+// 16% faster than with 8; 1M x 2M in FP32 (16 MB) with 8.
+constexpr int64_t kBandBytes = int64_t{2} << 20;
+inline PlanGeo single_geo(int64_t gathered_entries, int pair_bytes) {
+  PlanGeo g;
+  int per_xcd = 1;
+  while (per_xcd < 64 && gathered_entries * pair_bytes > int64_t{8} * per_xcd * kBandBytes) per_xcd *= 2;
+  g.bands = 8 * per_xcd;
+  return g;
+}
+
 // A stream item's rows are summed by L lanes each, L the largest power of two (at most 64) with
 // rows * L <= threads; one lane then walks ceil(len / L) products of a row serially. An item is
 // closed before a row that would take that past kSerialSum (twice what a single full-tile row
@@ -52,11 +66,16 @@ inline int lanes_per_row(int rows, int threads) {
 
 inline HostPlan build_plan(int32_t rows, int32_t ncols, const int32_t* rowptr, const int32_t* col,
                     std::vector<int4>* logical, const PlanGeo& g = PlanGeo{}) {
-  const int kXcds = g.bands;
+  // Column bands: g.bands of them, band b on XCD b % xcds; an XCD works through its bands in
+  // band order (all of band x's segments, then band x + xcds's, ...), so its L2 holds one band
+  // of the gathered vector at a time.
+  const int nbands = std::max(1, g.bands);
+  const int kXcds = std::min(nbands, 8);
   HostPlan hp;
   std::vector<int4> stream;
-  std::vector<std::vector<int4>> band(kXcds);
-  const int64_t width = std::max<int64_t>(1, (static_cast<int64_t>(ncols) + kXcds - 1) / kXcds);
+  std::vector<std::vector<int4>> band(kXcds);  // per XCD, its bands' segments in band order
+  std::vector<std::vector<int4>> by_band(nbands);
+  const int64_t width = std::max<int64_t>(1, (static_cast<int64_t>(ncols) + nbands - 1) / nbands);
   int32_t r = 0;
   while (r < rows) {
     const int32_t len = rowptr[r + 1] - rowptr[r];
@@ -65,14 +84,14 @@ inline HostPlan build_plan(int32_t rows, int32_t ncols, const int32_t* rowptr, c
       int32_t nseg = 0;
       int32_t k = rowptr[r];
       while (k < rowptr[r + 1]) {
-        const int b = static_cast<int>(std::min<int64_t>(col[k] / width, kXcds - 1));
+        const int b = static_cast<int>(std::min<int64_t>(col[k] / width, nbands - 1));
         const int64_t band_end_col = (b + 1) * width;
         int32_t kb = k;  // first nonzero of the next band
         while (kb < rowptr[r + 1] && col[kb] < band_end_col) ++kb;
         for (int32_t k0 = k; k0 < kb; k0 += g.seg) {
           const int32_t k1 = std::min(k0 + g.seg, kb);
           const int4 item = make_int4(hp.nseg + nseg, -lr - 1, k0, k1);
-          band[b].push_back(item);
+          by_band[b].push_back(item);
           logical->push_back(item);
           ++nseg;
         }
@@ -102,6 +121,8 @@ inline HostPlan build_plan(int32_t rows, int32_t ncols, const int32_t* rowptr, c
     stream.push_back(item);
     logical->push_back(item);
   }
+  for (int b = 0; b < nbands; ++b)
+    band[b % kXcds].insert(band[b % kXcds].end(), by_band[b].begin(), by_band[b].end());
   // Dispatch order: stream items spread evenly through the segments (a stream item is placed
   // whenever the share of stream items placed so far is at most the share of segments), so the
   // latency-bound short rows run beside the bandwidth-bound long-row segments instead of after

@@ -13,9 +13,14 @@ import sankhya_hip as sh
 
 
 def _hip():
-    """HIP's runtime as the engine library loaded it (torch bundles a second runtime of its own,
-    which does not see the device once this one holds it)."""
-    hip = C.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+    """HIP's runtime as the engine library bound it. The engine library is loaded first; then
+    the runtime is opened by its soname, which the dynamic loader resolves to the copy already
+    loaded under that name, the same rule that bound the engine library's own dependency: ROCm's
+    runtime when the engine came first, torch's bundled copy when torch was imported earlier in
+    the process (opening ROCm's by its path then would load a second runtime against torch's
+    older HSA library, and fail)."""
+    sh.load_library()
+    hip = C.CDLL("libamdhip64.so.7")
     hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     hip.hipFree.argtypes = [C.c_void_p]
     hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
