@@ -34,7 +34,7 @@ Other fields of the line:
     nonzeros per column) solved in one batch in mixed precision to relative KKT 1e-6, and again
     polished to 1e-8 in FP64 (pdhg_batch_promote), each against plain FP64 single-LP solves of a
     sample of the same LPs at the same tolerance; `speedup` is the ratio of the LPs/s;
-  - `adjoint`: 64 blend-style LPs (m = 5,000, n = 12,000) solved at relative KKT <= 1e-9, then
+  - `adjoint`: 64 blend-style LPs (m = 5,000, n = 12,000) solved toward relative KKT 2e-10, then
     the adjoint: forward and backward seconds, and the envelope error of L = c.x* (dL/db
     against y*) over the LPs with a square active system;
   - `roofline`: the dominant kernel's algorithmic bytes over its HIP-event-timed duration.
@@ -327,16 +327,19 @@ def c2_check(sh, iteration_cap=100_000, reps=20):
                 "objective_rel_diff_vs_fp64_1e6": rel(c), "status": c.status}}
 
 
-def adjoint_check(sh, count):
+def adjoint_check(sh, count, tolerance=2e-10, floor=1e-9):
     """The differentiable-LP workload: `count` blend LPs (m = 5,000, n = 12,000, seeds 5000+i)
-    solved in one batch at relative KKT <= 1e-9 (iteration limit 2,000,000), then the adjoint
-    (the backward of the torch layer) for L = c.x* and for L = w.x* (w ~ N(0,1)). Envelope check
-    of the first, dL/db against the row duals y*, relative to the largest entry (and, beside it,
-    in the 2-norm), on every LP that
-    converged and whose active system is square (free columns as many as rows with entries: an
-    empty row, 0 = 0, leaves A one rank short); a degenerate vertex (fewer free columns than
-    rows: a basic column at its bound) has no unique dual, and there the check is that dL/db is
-    an exact dual of the active set, A_F^T (dL/db) = c_F."""
+    solved in one batch asking relative KKT <= `tolerance` (iteration limit 2,000,000), then the
+    adjoint (the backward of the torch layer) for L = c.x* and for L = w.x* (w ~ N(0,1)).
+    Envelope check of the first, dL/db against the row duals y*, relative to the largest entry
+    (and, beside it, in the 2-norm), on every LP whose forward ended at relative KKT <= `floor`
+    (whether or not it reached `tolerance` within the limit) and whose active system is square
+    (free columns as many as rows with entries: an empty row, 0 = 0, leaves A one rank short);
+    a degenerate vertex (fewer free columns than rows: a basic column at its bound) has no
+    unique dual, and there the check is that dL/db is an exact dual of the active set,
+    A_F^T (dL/db) = c_F. The envelope error is the forward's own dual accuracy (the adjoint
+    solves the active system exactly): on this workload it was 1.0e-5 at worst with the forward
+    at 1e-9 and 1.8e-6 at 2e-10."""
     import numpy as np
     import scipy.sparse as sps
     import sankhya_hip.workloads as wl
@@ -346,12 +349,14 @@ def adjoint_check(sh, count):
     datas = [m.arrays() for m in models]
     log(f"[bench] adjoint workload: {count} blend LPs (generated in {time.perf_counter()-t:.1f}s)")
     t = time.perf_counter()
-    ses = sh.BatchSession(models, pdhg_tolerance=1e-9, pdhg_stop_rule="relative_kkt",
+    ses = sh.BatchSession(models, pdhg_tolerance=tolerance, pdhg_stop_rule="relative_kkt",
                           iteration_limit=2_000_000, pdhg_polish=False)
     tf = time.perf_counter() - t
     sols = ses.solutions
-    solved = [s.reached(1e-9) for s in sols]
-    log(f"[bench] adjoint forward: {sum(solved)}/{count} solved in {tf:.2f}s")
+    solved = [s.reached(floor) for s in sols]
+    reached = sum(1 for s in sols if s.reached(tolerance))
+    log(f"[bench] adjoint forward: {reached}/{count} at {tolerance:g}, {sum(solved)}/{count} at "
+        f"{floor:g} in {tf:.2f}s")
     t = time.perf_counter()
     gc, gb, its, _ = ses.adjoint([d["cost"] for d in datas], None)
     te = time.perf_counter() - t
@@ -386,8 +391,10 @@ def adjoint_check(sh, count):
     log(f"[bench] adjoint backward: c.x* {te:.2f}s, w.x* {tw:.2f}s; envelope error on {square} "
         f"square LPs {env_b:.1e}; {degenerate} degenerate (active-system residual {deg_resid:.1e})")
     return {"workload": f"blend-{count}: m = 5000, n = 12000, 40% of columns upper-bounded, "
-                        "relative KKT <= 1e-9, FP64 adjoint",
-            "lps": count, "solved": int(sum(solved)), "forward_seconds": tf,
+                        f"forward asked relative KKT <= {tolerance:g} within 2,000,000 "
+                        f"iterations, envelope checked where it ended <= {floor:g}; FP64 adjoint",
+            "lps": count, "reached_tolerance": reached, "solved": int(sum(solved)),
+            "forward_seconds": tf,
             "backward_seconds_objective": te, "backward_seconds_wx": tw,
             "cgls_iterations_median": int(np.median(its[:, 0])),
             "square_lps": square, "degenerate_lps": degenerate,

@@ -48,12 +48,22 @@ std::vector<OptionSpec> build_registry() {
   s.push_back({"pdhg_polish", OptionType::Bool, true,
                "Hand a PDHG point that is not optimal to the interior-point polish as a warm "
                "start (col_value, row_dual, col_dual). solve() runs it when an interior-point "
-               "engine is installed (sankhya::install_polish_engine); the engine itself never "
-               "reads this option.", 0, 0});
+               "engine is installed (sankhya::install_polish_engine, sankhya/ipm.hpp); the PDHG "
+               "engine itself never reads this option.", 0, 0});
   s.push_back({"pdhg_mixed_polish", OptionType::Bool, true,
                "Mixed precision: after the FP32 phase, continue the same run in FP64 on the GPU "
                "to pdhg_tolerance (the FP64 polish). Off: the FP32 phase runs to the end.", 0, 0});
-  s.push_back({"polish_iteration_limit", OptionType::Int, std::int64_t{50}, "Polish iterations.", 0.0, 9.0e18});
+  s.push_back({"polish_iteration_limit", OptionType::Int, std::int64_t{50},
+               "Interior-point iterations the polish may spend.", 1.0, 9.0e18});
+  s.push_back({"polish_max_seconds", OptionType::Double, 30.0,
+               "Wall-clock the polish may spend: the smaller of this and what time_limit has "
+               "left.", 0.1, kNoLimit});
+  s.push_back({"polish_start_margin", OptionType::Double, 1e-1,
+               "Read by the installed interior-point engine: the floor of its warm start's "
+               "slacks and multipliers.", 1e-8, 1.0});
+  s.push_back({"polish_max_factor_nonzeros", OptionType::Int, std::int64_t{50000000},
+               "Read by the installed interior-point engine: it declines the polish above this "
+               "many factor nonzeros; -1 for no cap.", -1.0, 9.0e18});
   s.push_back({"pdhg_tolerance", OptionType::Double, tol::kPdhgLoose, "PDHG relative KKT tolerance.", 0.0, 1.0});
   s.push_back({"pdhg_stop_at_request", OptionType::Bool, false,
                "Stop once pdhg_tolerance is met and the point is primal-feasible to the absolute "

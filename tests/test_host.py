@@ -242,3 +242,21 @@ def test_engine_sources_compile_against_the_reference_headers(source):
            "-I" + src, "-I" + fmt_include, os.path.join(src, source)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
+
+
+def test_interior_point_polish_handoff(tmp_path):
+    """solve()'s polish of a PDHG answer (the reference's polish_with_the_interior_point,
+    src/core/solve.cpp:239-308): tests/polish_handoff_test.cpp installs a stub interior-point
+    engine through sankhya::install_polish_engine and checks what it is handed (PDHG's point,
+    duals and reduced costs as the warm start; polish_iteration_limit; the time budget) and
+    what comes back (replaced only when better; iterations summed; algorithm pdhg-gpu+ipm)."""
+    inc = os.path.join(ROOT, "sih-26_amd", "include")
+    lib = os.path.join(ROOT, "sih-26_amd", "lib")
+    exe = str(tmp_path / "polish_handoff")
+    cmd = ["g++", "-std=c++17", "-O1", "-I" + inc, os.path.join(cm.HERE, "polish_handoff_test.cpp"),
+           "-L" + lib, "-lsankhya_hip", "-Wl,-rpath," + lib, "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "all checks passed" in r.stdout
