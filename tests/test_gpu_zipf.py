@@ -14,7 +14,9 @@ itself, with one cost entry moved by one ulp, parts by 1e-2 relative in its resi
 1,227,840 iterations unperturbed, past 2,000,000 (the limit) for each of three one-ulp moves.
 The engine sums in a different order from the oracle in every product, so it is held to the
 outcomes rounding allows (the fixture holds the perturbed solves, key suffix -ulp...):
-  - a status the oracle itself ends with, unperturbed or under a one-ulp move;
+  - a status the oracle itself ends with, unperturbed or under a one-ulp move, or else a
+    claim (optimal, feasible) where the oracle ran out of iterations, which the checks below
+    then confirm on the engine's own point (a path that rounds differently may stop sooner);
   - a point that the oracle's evaluate() (the reference's src/pdhg/pdhg_evaluate.cpp:19-139)
     finds to meet the stopping test that status claims;
   - the objective within the weak-duality envelopes of the engine's point and the oracle's
@@ -86,7 +88,7 @@ def check_against(rec, perturbed, s, d):
     _, _, e = orc.evaluate(P, s.col_value, -s.row_dual)
     oe = np.asarray(rec["final_residuals"])
     allowed = {rec["status"]} | {p["status"] for p in perturbed}
-    assert s.status in allowed, (s.status, allowed, s.message)
+    assert s.status in allowed or s.status in ("optimal", "feasible"), (s.status, allowed, s.message)
     rel = abs(s.objective - rec["objective"]) / max(1.0, abs(rec["objective"]))
     if s.status == "optimal":
         assert meets_standard(e) and meets_request(e, rec["tolerance"]), e
