@@ -484,6 +484,7 @@ __global__ __launch_bounds__(kCT) void k_coop_tail(CoopArgs<V, X> C) {
     a.s[0] = p[0];
     a.s[1] = q[0];
     a.s[2] = q[1];
+    return true;
   };
   Acc<2, 0> lacc;
   lacc.zero();

@@ -55,9 +55,12 @@ __global__ __launch_bounds__(kThreads) void k_primal(PlanView<V> T, Iter<X> it, 
   __shared__ double lds[kTile];
   __shared__ double red[4 * 2];
   const int4 item = T.items[blockIdx.x];
-  if (!force && ctl->halt) return;
-  PrimalGather<X> gather{it.ybuf, ctl->cur};
-  PrimalEpi<X> epi = primal_epi(it, ctl);
+  const AttemptCtl c = read_attempt(ctl);
+  if (attempt_halted(c, force, item,
+                     iter_bits(it) | plan_bits(T) | reinterpret_cast<uintptr_t>(part)))
+    return;
+  PrimalGather<X> gather{it.ybuf, c.cur};
+  PrimalEpi<X> epi = primal_epi(it, c);
   Acc<1, 0> acc;
   acc.zero();
   run_item<V, 1>(T, item, gather, epi, acc, lds, red);
@@ -71,11 +74,13 @@ __global__ __launch_bounds__(kTail) void k_primal_tail(PlanView<V> T, Iter<X> it
                                                        double* lr_sum, int force) {
   __shared__ double lds[kChunk];
   __shared__ double red[16 * 1];
-  if (!force && ctl->halt) return;
-  PrimalEpi<X> epi = primal_epi(it, ctl);
+  // The long columns' inputs were staged by k_primal: the halt test waits on no other load.
+  const AttemptCtl c = read_attempt(ctl);
+  const int halt = force ? 0 : c.halt;
+  PrimalEpi<X> epi = primal_epi(it, c);
   Acc<1, 0> acc;
   acc.zero();
-  finish_long_rows<V, 1>(T, epi, acc, lds);
+  if (!finish_staged_rows<V, 1>(T, epi, acc, lds, [&]() { return halt == 0; })) return;
   block_reduce(acc, red);
   if (threadIdx.x == 0) lr_sum[0] = acc.s[0];
 }
@@ -88,14 +93,12 @@ __global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ct
   __shared__ double lds[kTile];
   __shared__ double red[4 * 2];
   const int4 item = A.items[blockIdx.x];
-  if (!force && ctl->halt) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const StepFactors f = step_factors(ctl->iteration);
-    stepf[0] = f.shrink;
-    stepf[1] = f.grow;
-  }
-  DualGather<X> gather{it.xbuf, ctl->cur};
-  DualEpi<X> epi = dual_epi(it, ctl);
+  const AttemptCtl c = read_attempt(ctl);
+  if (attempt_halted(c, force, item,
+                     iter_bits(it) | plan_bits(A) | reinterpret_cast<uintptr_t>(part)))
+    return;
+  DualGather<X> gather{it.xbuf, c.cur};
+  DualEpi<X> epi = dual_epi(it, c);
   Acc<2, 0> acc;
   acc.zero();
   run_item<V, 2>(A, item, gather, epi, acc, lds, red);
@@ -104,46 +107,58 @@ __global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ct
     part[2 * blockIdx.x] = acc.s[0];
     part[2 * blockIdx.x + 1] = acc.s[1];
   }
+  // After its own item, so that the iteration count's load and the two pow calls do not
+  // delay the item's loads.
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const StepFactors f = step_factors(ctl->iteration);
+    stepf[0] = f.shrink;
+    stepf[1] = f.grow;
+  }
 }
 
 // Completes the long rows of Ahat, then adds every partial in index order and takes the
 // step decision: one workgroup, after k_primal(+tail) and k_dual.
 template <typename V, typename X>
 __global__ __launch_bounds__(kTail) void k_dual_tail(PlanView<V> A, Iter<X> it, Ctl* ctl,
-                                                     const double* part_p, int np,
+                                                     const double* part_p,
                                                      const double* lr_p, int has_lr_p,
-                                                     const double* part_d, int nd,
+                                                     const double* part_d, int nd, int rounds,
                                                      const double* stepf, int force) {
   __shared__ double lds[2 * kChunk];
   __shared__ double red[16 * 5];
-  if (!force && ctl->halt) return;
-  DualEpi<X> epi = dual_epi(it, ctl);
-  StepSnap snap{};
-  StepFactors f{};
-  if (threadIdx.x == 0) {
-    snap = step_snap(ctl);
-    // Workgroup 0 of k_dual wrote them; with no dual items they are evaluated here.
-    f = nd > 0 ? StepFactors{stepf[0], stepf[1]} : step_factors(snap.iteration);
-  }
+  // Every load of this kernel is independent of every other (the long rows' inputs were
+  // staged by k_dual), so all of them, the halt flag included, share one memory round trip:
+  // the halt test comes after they are issued and before anything is written. The control
+  // fields are read by every thread (uniform scalar loads, no branch to wait at).
+  const AttemptCtl c = read_attempt(ctl);
+  const int halt = force ? 0 : c.halt;
+  DualEpi<X> epi = dual_epi(it, c);
+  const StepSnap snap = step_snap(ctl);
+  // Workgroup 0 of k_dual wrote them; with no dual items they are evaluated here.
+  const double f0 = stepf[0], f1 = stepf[1];
   // s[0] primal movement, s[1] dual movement, s[2] interaction (per-workgroup partials);
   // s[3], s[4] the same two dual terms from the long rows.
   Acc<5, 0> a;
   a.zero();
   auto partials = [&]() {
     double p[1] = {0.0}, d[2] = {0.0, 0.0};
-    lane_sums_1_2(part_p, np, part_d, nd, p, d);
+    lane_sums_1_2_padded(part_p, part_d, rounds, p, d);
     if (has_lr_p && threadIdx.x == 0) p[0] += lr_p[0];
     a.s[0] = p[0];
     a.s[1] = d[0];
     a.s[2] = d[1];
+    return halt == 0;
   };
   Acc<2, 0> lacc;
   lacc.zero();
-  finish_long_rows<V, 2>(A, epi, lacc, lds, partials);
+  if (!finish_staged_rows<V, 2>(A, epi, lacc, lds, partials)) return;
   a.s[3] = lacc.s[0];
   a.s[4] = lacc.s[1];
   block_reduce(a, red);
-  if (threadIdx.x == 0) step_decide(ctl, snap, a.s[0], a.s[1] + a.s[3], a.s[2] + a.s[4], f);
+  if (threadIdx.x == 0) {
+    const StepFactors f = nd > 0 ? StepFactors{f0, f1} : step_factors(snap.iteration);
+    step_decide(ctl, snap, a.s[0], a.s[1] + a.s[3], a.s[2] + a.s[4], f);
+  }
 }
 
 template <typename V, typename X>
@@ -549,6 +564,7 @@ struct DevPlan {
   int32_t ngroups = 0;
   int4 group0 = make_int4(0, 0, 0, 0);  // host copy of lr_groups[0]
   double* seg_part = nullptr;
+  double* lr_in = nullptr;  // staged epilogue inputs of the long rows (kStage per long row)
   double* raw64 = nullptr;  // A's own values in this layout, kept for pdhg_dev_precondition
   template <typename V>
   PlanView<V> view() const {
@@ -566,6 +582,13 @@ struct DevPlan {
     p.ngroups = ngroups;
     p.group0 = group0;
     p.has_group0 = ngroups > 0 ? 1 : 0;
+    return p;
+  }
+  // The view of an iteration kernel and its tail, which stage the long rows' epilogue inputs.
+  template <typename V>
+  PlanView<V> staged_view() const {
+    PlanView<V> p = view<V>();
+    p.lr_in = lr_in;
     return p;
   }
 };
@@ -719,6 +742,7 @@ int make_plan(pdhg_dev* h, DevPlan* P, int32_t rows, int32_t ncols, int64_t nnz,
   P->ngroups = static_cast<int32_t>(hp.groups.size());
   if (!hp.groups.empty()) P->group0 = hp.groups[0];
   if ((rc = dalloc(h, &P->seg_part, static_cast<size_t>(hp.nseg) * 2))) return rc;
+  if ((rc = dalloc(h, &P->lr_in, static_cast<size_t>(P->nlr) * kStage))) return rc;
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return PDHG_OK;
 }
@@ -758,6 +782,15 @@ Iter<X> iter_of(const pdhg_dev* h) {
   return it;
 }
 
+// Rounds of k_dual_tail's unconditional partial-sum loads (lane_sums_1_2_padded): the buffers
+// part_p (one partial per item of Ahat^T) and part_d (two per item of Ahat) hold that many
+// rounds of kSumU * kTail entries, zero past the items.
+int sum_rounds(int64_t items_a, int64_t items_t) {
+  const int64_t most = std::max<int64_t>(std::max(items_a, items_t), 1);
+  constexpr int64_t kRound = int64_t{kSumU} * kTail;
+  return static_cast<int>((most + kRound - 1) / kRound);
+}
+
 int apply_grid(const pdhg_dev* h) {
   const int64_t w = std::max<int64_t>(std::max<int64_t>(h->n, h->m), 1);
   return static_cast<int>(std::min<int64_t>((w + kThreads - 1) / kThreads, 8192));
@@ -767,16 +800,18 @@ template <typename V, typename X>
 void launch_attempt(pdhg_dev* h, int force) {
   const Iter<X> it = iter_of<X>(h);
   if (h->T.nitems > 0)
-    k_primal<V, X><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<V>(), it, h->ctl, h->part_p,
-                                                            force);
+    k_primal<V, X><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.staged_view<V>(), it, h->ctl,
+                                                            h->part_p, force);
   if (h->T.nlr > 0)
-    k_primal_tail<V, X><<<1, kTail, 0, h->stream>>>(h->T.view<V>(), it, h->ctl, h->lr_p, force);
+    k_primal_tail<V, X><<<1, kTail, 0, h->stream>>>(h->T.staged_view<V>(), it, h->ctl, h->lr_p,
+                                                    force);
   if (h->A.nitems > 0)
-    k_dual<V, X><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<V>(), it, h->ctl, h->part_d,
-                                                          h->stepf, force);
-  k_dual_tail<V, X><<<1, kTail, 0, h->stream>>>(h->A.view<V>(), it, h->ctl, h->part_p, h->T.nitems,
+    k_dual<V, X><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.staged_view<V>(), it, h->ctl,
+                                                          h->part_d, h->stepf, force);
+  k_dual_tail<V, X><<<1, kTail, 0, h->stream>>>(h->A.staged_view<V>(), it, h->ctl, h->part_p,
                                                 h->lr_p, h->T.nlr > 0 ? 1 : 0, h->part_d,
-                                                h->A.nitems, h->stepf, force);
+                                                h->A.nitems, sum_rounds(h->A.nitems, h->T.nitems),
+                                                h->stepf, force);
 }
 
 // Anchor products Ahat xrest and Ahat^T yrest in FP64 (delta form), gated on a restart.
@@ -1031,7 +1066,7 @@ int64_t pdhg_dev_estimate_bytes(int32_t m, int32_t n, int64_t nnz, const int32_t
     const int64_t ni = static_cast<int64_t>(hp.items.size()), nl = static_cast<int64_t>(hp.lr_nseg.size());
     total += bytes(rows + 1, 4) + bytes(nnz, 4) + bytes(nnz, 8) + (mx ? bytes(nnz, 4) : 0);
     total += bytes(ni, 16) + bytes(nl, 16) + bytes(static_cast<int64_t>(hp.groups.size()), 16);
-    total += bytes(2 * static_cast<int64_t>(hp.nseg), 8);
+    total += bytes(2 * static_cast<int64_t>(hp.nseg), 8) + bytes(nl * kStage, 8);
     if (unscaled && keep_unscaled) total += bytes(nnz, 8);  // raw64
     (side == 0 ? items_a : items_t) = static_cast<int32_t>(ni);
   }
@@ -1041,7 +1076,9 @@ int64_t pdhg_dev_estimate_bytes(int32_t m, int32_t n, int64_t nnz, const int32_t
   total += 6 * bytes(n, 8) + 6 * bytes(m, 8);                         // sums, restart, best, scratch
   total += 7 * bytes(n, 8) + 5 * bytes(m, 8);                         // problem vectors
   const int64_t ia = std::max(items_a, 1), it = std::max(items_t, 1);
-  total += bytes(it, 8) + bytes(1, 8) + bytes(2 * ia, 8) + bytes(1, 8) + bytes(9 * ia, 8) + bytes(9 * it, 8);
+  const int64_t padded = int64_t{sum_rounds(items_a, items_t)} * kSumU * kTail;
+  total += bytes(padded, 8) + bytes(1, 8) + bytes(2 * padded, 8) + bytes(1, 8) + bytes(9 * ia, 8) +
+           bytes(9 * it, 8);
   total += bytes(1, sizeof(Ctl)) + bytes(kTickCap, sizeof(pdhg_tick)) + bytes(2, 8);
   if (unscaled) total += bytes(2 * 1024 + 2, 8);  // preconditioner scratch
   return total;
@@ -1117,12 +1154,16 @@ int pdhg_dev_upload(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz, const int32_
     return rc;
   const size_t ia = static_cast<size_t>(std::max(h->A.nitems, 1));
   const size_t it = static_cast<size_t>(std::max(h->T.nitems, 1));
-  if ((rc = dalloc(h, &h->part_p, it)) || (rc = dalloc(h, &h->lr_p, 1)) ||
-      (rc = dalloc(h, &h->part_d, 2 * ia)) || (rc = dalloc(h, &h->lr_sum, 1)) ||
+  const size_t padded = static_cast<size_t>(sum_rounds(h->A.nitems, h->T.nitems)) * kSumU * kTail;
+  if ((rc = dalloc(h, &h->part_p, padded)) || (rc = dalloc(h, &h->lr_p, 1)) ||
+      (rc = dalloc(h, &h->part_d, 2 * padded)) || (rc = dalloc(h, &h->lr_sum, 1)) ||
       (rc = dalloc(h, &h->part_er, 9 * ia)) || (rc = dalloc(h, &h->part_ec, 9 * it)) ||
       (rc = dalloc(h, &h->ctl, 1)) || (rc = dalloc(h, &h->ticks, static_cast<size_t>(kTickCap))) ||
       (rc = dalloc(h, &h->stepf, 2)))
     return rc;
+  // The padding past the items must read zero (k_dual_tail adds it).
+  HIP_TRY(h, hipMemsetAsync(h->part_p, 0, padded * sizeof(double), h->stream));
+  HIP_TRY(h, hipMemsetAsync(h->part_d, 0, 2 * padded * sizeof(double), h->stream));
   Ctl c;
   std::memset(&c, 0, sizeof(c));
   c.bound_norm = bound_norm;
@@ -1465,13 +1506,13 @@ int pdhg_dev_time_kernels(pdhg_dev* h, int reps, double* primal_ms, double* dual
   const Iter<float> itf = iter_of<float>(h);
   HIP_TRY(h, hipEventRecord(e0, h->stream));
   for (int r = 0; r < reps && h->T.nitems > 0; ++r) {
-    if (mixed(h)) k_primal<float, float><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<float>(), itf, h->ctl, h->part_p, 1);
-    else k_primal<double, double><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<double>(), itd, h->ctl, h->part_p, 1);
+    if (mixed(h)) k_primal<float, float><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.staged_view<float>(), itf, h->ctl, h->part_p, 1);
+    else k_primal<double, double><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.staged_view<double>(), itd, h->ctl, h->part_p, 1);
   }
   HIP_TRY(h, hipEventRecord(e1, h->stream));
   for (int r = 0; r < reps && h->A.nitems > 0; ++r) {
-    if (mixed(h)) k_dual<float, float><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<float>(), itf, h->ctl, h->part_d, h->stepf, 1);
-    else k_dual<double, double><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<double>(), itd, h->ctl, h->part_d, h->stepf, 1);
+    if (mixed(h)) k_dual<float, float><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.staged_view<float>(), itf, h->ctl, h->part_d, h->stepf, 1);
+    else k_dual<double, double><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.staged_view<double>(), itd, h->ctl, h->part_d, h->stepf, 1);
   }
   HIP_TRY(h, hipEventRecord(e2, h->stream));
   HIP_TRY(h, hipEventSynchronize(e2));

@@ -391,6 +391,54 @@ template <typename X>
 __device__ DualEpi<X> dual_epi(const Iter<X>& it, const Ctl* ctl) {
   return DualEpi<X>{it, ctl->cur, ctl->pending, ctl->eta * ctl->omega, ctl->omega};
 }
+
+// The control fields an attempt kernel reads, taken in one step at kernel start. The attempt
+// kernels test `halt` before any work, and a workgroup must not wait for that test before
+// issuing its other first loads (its work item, the kernel's own arguments): read_attempt keeps
+// every one of these fields live across the test, so they are all loaded, and waited for, once.
+struct AttemptCtl {
+  double eta, omega;
+  int32_t cur, pending, halt;
+};
+__device__ __forceinline__ AttemptCtl read_attempt(const Ctl* ctl) {
+  return AttemptCtl{ctl->eta, ctl->omega, ctl->cur, ctl->pending, ctl->halt};
+}
+// True when the attempt kernels must exit at once. The other terms never hold (no work item
+// has k1 < k0; omega and eta are positive, or NaN, which compares false; cur and pending are 0
+// or 1): they make the test read the work item and every control field, so the compiler issues
+// all those loads before the test instead of after it.
+// `args` folds the kernel's argument pointers the same way (device allocations are 256-byte
+// aligned, so their low bits are zero): their loads then go out with the first ones too.
+__device__ __forceinline__ bool attempt_halted(const AttemptCtl& c, int force, const int4& item,
+                                               uintptr_t args = 0) {
+  // Bitwise, not short-circuit: every operand is evaluated, so no load waits behind a branch.
+  const int never = static_cast<int>(item.w < item.z) | static_cast<int>(c.omega < 0.0) |
+                    static_cast<int>(c.eta < -1.0) | static_cast<int>(c.cur < 0) |
+                    static_cast<int>(c.pending < 0) | static_cast<int>((args & 7) != 0);
+  const int halt = static_cast<int>(force == 0) & static_cast<int>(c.halt != 0);
+  return (halt | never) != 0;
+}
+template <typename X>
+__device__ __forceinline__ uintptr_t iter_bits(const Iter<X>& it) {
+  auto b = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
+  uintptr_t r = b(it.xbuf) | b(it.ybuf) | b(it.xsum) | b(it.ysum) | b(it.chat) | b(it.lhat) |
+                b(it.uhat) | b(it.rlhat) | b(it.ruhat);
+  if constexpr (kDelta<X>) r |= b(it.xrest) | b(it.yrest) | b(it.axr) | b(it.atyr);
+  return r;
+}
+template <typename V>
+__device__ __forceinline__ uintptr_t plan_bits(const PlanView<V>& P) {
+  return reinterpret_cast<uintptr_t>(P.lr_info) | reinterpret_cast<uintptr_t>(P.seg_part) |
+         reinterpret_cast<uintptr_t>(P.lr_in);
+}
+template <typename X>
+__device__ PrimalEpi<X> primal_epi(const Iter<X>& it, const AttemptCtl& c) {
+  return PrimalEpi<X>{it, c.cur, c.pending, c.eta / c.omega, c.omega};
+}
+template <typename X>
+__device__ DualEpi<X> dual_epi(const Iter<X>& it, const AttemptCtl& c) {
+  return DualEpi<X>{it, c.cur, c.pending, c.eta * c.omega, c.omega};
+}
 __device__ inline double tick_count(const Ctl* ctl) {
   return static_cast<double>(ctl->averaged > 0 ? ctl->averaged : 1);
 }

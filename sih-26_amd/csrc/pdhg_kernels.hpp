@@ -78,15 +78,58 @@ __device__ __forceinline__ double2 load_vals2(const float* p) {
   return make_double2(static_cast<double>(f.x), static_cast<double>(f.y));
 }
 
+// Cross-lane steps of the butterfly reductions, smallest offset first. After the steps below
+// `half`, every aligned group of `half` lanes holds one value, so any lane of the other half of
+// the aligned 2*half group is as good a partner as lane ^ half: IEEE addition and smax are
+// commutative, so the result is the one an exact xor exchange gives. That lets the steps within
+// a row of 16 lanes use DPP lane moves (quad_perm for 1 and 2, row_half_mirror for 4, row_mirror
+// for 8), a few cycles each, instead of a round trip through the LDS crossbar.
+template <int HALF>
+__device__ __forceinline__ double dpp_partner(double v) {
+  static_assert(HALF == 1 || HALF == 2 || HALF == 4 || HALF == 8, "DPP steps stay in a row");
+  constexpr int kCtrl = HALF == 1 ? 0xB1 : HALF == 2 ? 0x4E : HALF == 4 ? 0x141 : 0x140;
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), kCtrl, 0xF, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<unsigned>(lo)));
+}
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<unsigned>(lo)));
+}
+
+// Sum over each aligned group of L lanes (L a power of two, at most 64; uniform), which every
+// lane of the group receives. The lanes of one group are all active or all inactive.
+__device__ __forceinline__ double group_sum(double t, int L) {
+  if (L >= 2) t += dpp_partner<1>(t);
+  if (L >= 4) t += dpp_partner<2>(t);
+  if (L >= 8) t += dpp_partner<4>(t);
+  if (L >= 16) t += dpp_partner<8>(t);
+  if (L >= 32) t += __shfl_xor(t, 16);
+  if (L >= 64) t += __shfl_xor(t, 32);
+  return t;
+}
+
+// Whole-wavefront sum and maximum (every lane active), returned to every lane: DPP within each
+// row of 16 lanes, then the four row results read as scalars and combined as the xor-16 and
+// xor-32 steps would combine them.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
+  v += dpp_partner<1>(v);
+  v += dpp_partner<2>(v);
+  v += dpp_partner<4>(v);
+  v += dpp_partner<8>(v);
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = smax(v, __shfl_xor(v, off));
-  return v;
+  v = smax(v, dpp_partner<1>(v));
+  v = smax(v, dpp_partner<2>(v));
+  v = smax(v, dpp_partner<4>(v));
+  v = smax(v, dpp_partner<8>(v));
+  return smax(smax(readlane_d(v, 0), readlane_d(v, 16)), smax(readlane_d(v, 32), readlane_d(v, 48)));
 }
 
 // Fixed-shape reduction of NS sums and NM maxima over one workgroup; result in thread 0.
@@ -102,8 +145,12 @@ struct Acc {
   }
 };
 
+// The wave results go through LDS; lane k of wave 0 then combines value k over the waves (in
+// wave order), all NS + NM values at once, and thread 0 reads them from those lanes.
 template <int NS, int NM>
 __device__ void block_reduce(Acc<NS, NM>& a, double* red /* >= waves*(NS+NM) doubles of LDS */) {
+  constexpr int NV = NS + NM;
+  static_assert(NV <= 64, "one lane per value");
   const int nwaves = static_cast<int>(blockDim.x) >> 6;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -113,24 +160,26 @@ __device__ void block_reduce(Acc<NS, NM>& a, double* red /* >= waves*(NS+NM) dou
   __syncthreads();
   if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < NS; ++k) red[wave * (NS + NM) + k] = a.s[k];
+    for (int k = 0; k < NS; ++k) red[wave * NV + k] = a.s[k];
 #pragma unroll
-    for (int k = 0; k < NM; ++k) red[wave * (NS + NM) + NS + k] = a.m[k];
+    for (int k = 0; k < NM; ++k) red[wave * NV + NS + k] = a.m[k];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      double t = red[k];
-      for (int w = 1; w < nwaves; ++w) t += red[w * (NS + NM) + k];
-      a.s[k] = t;
+  if (wave == 0) {
+    double t = 0.0;
+    if (lane < NV) {
+      const bool is_max = lane >= NS;
+      t = red[lane];
+#pragma unroll 4
+      for (int w = 1; w < nwaves; ++w) {
+        const double v = red[w * NV + lane];
+        t = is_max ? smax(t, v) : t + v;
+      }
     }
 #pragma unroll
-    for (int k = 0; k < NM; ++k) {
-      double t = red[NS + k];
-      for (int w = 1; w < nwaves; ++w) t = smax(t, red[w * (NS + NM) + NS + k]);
-      a.m[k] = t;
-    }
+    for (int k = 0; k < NS; ++k) a.s[k] = readlane_d(t, k);
+#pragma unroll
+    for (int k = 0; k < NM; ++k) a.m[k] = readlane_d(t, NS + k);
   }
   __syncthreads();
 }
@@ -217,7 +266,37 @@ struct PlanView {
   // not load it before their first segment sums: one memory round trip less per launch.
   int4 group0;
   int32_t has_group0;
+  // Staged epilogue inputs, kStage doubles per long row, or null. When set, the workgroup of a
+  // long row's first segment loads that row's epilogue inputs beside its own work and stores
+  // them here, and the tail kernel reads them by long-row index: the tail then needs no load
+  // that waits on another (the row id from lr_info, then the row's inputs), one memory round
+  // trip less on every attempt's critical path. Valid only where the epilogue's inputs do not
+  // change between the product kernel and its tail kernel.
+  double* lr_in = nullptr;
 };
+
+constexpr int kStage = 8;  // doubles staged per long row (every epilogue's In fits)
+
+// An epilogue input record kept as doubles.
+template <class In>
+constexpr bool kStageable = sizeof(In) >= sizeof(double) && sizeof(In) % sizeof(double) == 0 &&
+                            sizeof(In) <= kStage * sizeof(double);
+template <class In>
+__device__ __forceinline__ void stage_store(double* p, const In& in) {
+  double v[sizeof(In) / sizeof(double)];
+  __builtin_memcpy(v, &in, sizeof(In));
+#pragma unroll
+  for (int k = 0; k < static_cast<int>(sizeof(In) / sizeof(double)); ++k) p[k] = v[k];
+}
+template <class In>
+__device__ __forceinline__ In stage_load(const double* p) {
+  double v[sizeof(In) / sizeof(double)];
+#pragma unroll
+  for (int k = 0; k < static_cast<int>(sizeof(In) / sizeof(double)); ++k) v[k] = p[k];
+  In r;
+  __builtin_memcpy(&r, v, sizeof(In));
+  return r;
+}
 
 template <int NR>
 struct RowSum {
@@ -344,7 +423,7 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
         double t = 0.0;
         if (rr < rows) {
           for (int s = rs[q] + lane; s < re[q]; s += L) t += lds[s];
-          for (int off = L >> 1; off > 0; off >>= 1) t += __shfl_xor(t, off);
+          t = group_sum(t, L);
         }
         sum[q].v[r] = t;
       }
@@ -362,6 +441,13 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
   Acc<NR, 0> seg;
 #pragma unroll
   for (int r = 0; r < NR; ++r) seg.s[r] = 0.0;
+  // Staging (PlanView::lr_in): the long row's descriptor is read by every segment workgroup
+  // (one uniform load beside the nonzeros, unconditional, so nothing waits for it early); the
+  // first segment's workgroup loads the row's epilogue inputs once its products are formed and
+  // stores them after the reduction.
+  constexpr bool kStaged = kStageable<typename E::In>;
+  const int lr = -it.y - 1;
+  const int4 info = P.lr_info[lr];
   {
     const int kb = k0 & ~1;
     constexpr int U = kSeg / (2 * kThreads) + 1;
@@ -408,10 +494,18 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
       }
     }
   }
+  const bool stager = kStaged && P.lr_in != nullptr && tid == 0 && seg_id == info.y;
+  typename E::In staged{};
+  if constexpr (kStaged) {
+    if (stager) staged = epi.load(info.x);
+  }
   block_reduce(seg, red);
   if (tid == 0) {
 #pragma unroll
     for (int r = 0; r < NR; ++r) P.seg_part[(int64_t)seg_id * NR + r] = seg.s[r];
+  }
+  if constexpr (kStaged) {
+    if (stager) stage_store(P.lr_in + static_cast<int64_t>(lr) * kStage, staged);
   }
 }
 
@@ -423,74 +517,109 @@ __device__ __forceinline__ void run_item(const PlanView<V>& P, const int4 it, G&
 
 // Completes every long row of P in one workgroup (launched after the kernel that wrote the
 // segment sums): for each group of at most blockDim long rows, stage their segment sums in
-// LDS with coalesced loads while each row's lane fetches the row's epilogue inputs; then a
-// power-of-two group of lanes per row adds its segments in index order and one lane runs epi.
+// LDS with coalesced loads while each row's lane fetches the row's epilogue inputs (from
+// P.lr_in when the product kernel staged them); then a power-of-two group of lanes per row adds
+// its segments in index order and one lane runs epi.
 // `pre()` runs once, after the first group's loads are issued and before they are used, so a
-// caller's own independent loads share that memory round trip.
+// caller's own independent loads share that memory round trip. It returns false to end the
+// walk before anything is written (the same answer in every thread); finish_long_rows_geo then
+// returns false.
 struct NoPre {
-  __device__ void operator()() const {}
+  __device__ bool operator()() const { return true; }
 };
 
-template <int NTAIL, typename V, int NR, int NS, int NM, class E, class Pre = NoPre>
-__device__ void finish_long_rows_geo(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
-                                     double* lds /* kChunk * NR */, Pre pre = Pre()) {
-  bool pre_done = false;
-  for (int g = 0; g < P.ngroups; ++g) {
-    const int4 grp = (g == 0 && P.has_group0) ? P.group0 : P.lr_groups[g];  // {first, last,
-                                                                             // base0, count}
-    const int nrows = grp.y - grp.x, base0 = grp.z, cnt = grp.w;
-    int L = 64;
-    while (L > 1 && nrows * L > static_cast<int>(blockDim.x)) L >>= 1;
-    const int gi = threadIdx.x / L, lane = threadIdx.x & (L - 1);
-    const int lr = grp.x + gi;  // at most one row per lane group: nrows <= blockDim / L
-    const bool mine = gi < nrows;
-    int4 info = make_int4(0, 0, 0, 0);
+// One group of long rows {first, last) long rows, first segment, segment count}: the segment
+// sums, row descriptors and epilogue inputs are all loaded, then `hook()`, then LDS, sums and
+// epilogues. Returns false (having written nothing) when hook() does. With kFromStage the
+// epilogue inputs come from P.lr_in (the product kernel staged them) and every load is
+// unconditional at a clamped index, so no load waits for another and nothing waits before all
+// of them are issued.
+template <int NTAIL, typename V, int NR, bool kFromStage, int NS, int NM, class E, class Hook>
+__device__ __forceinline__ bool long_row_group(const PlanView<V>& P, const int4 grp, E& epi,
+                                               Acc<NS, NM>& acc, double* lds, Hook& hook) {
+  static_assert(!kFromStage || kStageable<typename E::In>, "staged inputs need a stageable In");
+  const int nrows = grp.y - grp.x, base0 = grp.z, cnt = grp.w;
+  int L = 64;
+  while (L > 1 && nrows * L > static_cast<int>(blockDim.x)) L >>= 1;
+  const int gi = threadIdx.x / L, lane = threadIdx.x & (L - 1);
+  const bool mine = gi < nrows;  // at most one row per lane group: nrows <= blockDim / L
+  const int lr = grp.x + (mine ? gi : 0);
+  // kChunk * NR / NTAIL loads per lane at most, held in registers until `hook` has run.
+  constexpr int U = (kChunk * NR + NTAIL - 1) / NTAIL;
+  double v[U];
+  int4 info;
+  typename E::In ein;
+  if constexpr (kFromStage) {
+    info = P.lr_info[lr];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = u * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+      const double x = P.seg_part[(int64_t)base0 * NR + (t < cnt * NR ? t : 0)];
+      v[u] = t < cnt * NR ? x : 0.0;
+    }
+    ein = stage_load<typename E::In>(P.lr_in + static_cast<int64_t>(lr) * kStage);
+  } else {
+    info = make_int4(0, 0, 0, 0);
     if (mine) info = P.lr_info[lr];
-    // kChunk * NR / NTAIL loads per lane at most, held in registers until `pre` has run.
-    constexpr int U = (kChunk * NR + NTAIL - 1) / NTAIL;
-    double v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int t = u * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
       v[u] = t < cnt * NR ? P.seg_part[(int64_t)base0 * NR + t] : 0.0;
     }
-    typename E::In ein;
     if (mine && lane == 0) ein = epi.load(info.x);
-    if (!pre_done) {
-      pre();
-      pre_done = true;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int t = u * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
-      if (t < cnt * NR) lds[t] = v[u];
-    }
-    __syncthreads();
-    if (mine) {
-      const int s0 = info.y - base0, ns = info.z;
-      RowSum<NR> sum;
-#pragma unroll
-      for (int r = 0; r < NR; ++r) sum.v[r] = 0.0;
-      for (int s = lane; s < ns; s += L) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) sum.v[r] += lds[(s0 + s) * NR + r];
-      }
-      for (int off = L >> 1; off > 0; off >>= 1) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) sum.v[r] += __shfl_xor(sum.v[r], off);
-      }
-      if (lane == 0) epi(info.x, sum, ein, acc);
-    }
-    __syncthreads();
   }
-  if (!pre_done) pre();
+  if (!hook()) return false;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int t = u * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (t < cnt * NR) lds[t] = v[u];
+  }
+  __syncthreads();
+  if (mine) {
+    const int s0 = info.y - base0, ns = info.z;
+    RowSum<NR> sum;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) sum.v[r] = 0.0;
+#pragma unroll 4
+    for (int s = lane; s < ns; s += L) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) sum.v[r] += lds[(s0 + s) * NR + r];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) sum.v[r] = group_sum(sum.v[r], L);
+    if (lane == 0) epi(info.x, sum, ein, acc);
+  }
+  __syncthreads();
+  return true;
+}
+
+// kFromStage: the single-LP engine's iteration tails, whose product kernels staged the long
+// rows' epilogue inputs (PlanView::lr_in) and whose plans carry their first group by value.
+// The first group is walked outside the loop, so its loads are issued at kernel start.
+template <int NTAIL, typename V, int NR, bool kFromStage = false, int NS, int NM, class E,
+          class Pre = NoPre>
+__device__ bool finish_long_rows_geo(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
+                                     double* lds /* kChunk * NR */, Pre pre = Pre()) {
+  if (P.ngroups <= 0) return pre();
+  const int4 g0 = (kFromStage || P.has_group0) ? P.group0 : P.lr_groups[0];
+  if (!long_row_group<NTAIL, V, NR, kFromStage>(P, g0, epi, acc, lds, pre)) return false;
+  NoPre none;
+  for (int g = 1; g < P.ngroups; ++g)
+    long_row_group<NTAIL, V, NR, kFromStage>(P, P.lr_groups[g], epi, acc, lds, none);
+  return true;
 }
 
 // The single-LP engine's tail kernels run kTail threads.
 template <typename V, int NR, int NS, int NM, class E, class Pre = NoPre>
-__device__ __forceinline__ void finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
+__device__ __forceinline__ bool finish_long_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
                                                  double* lds, Pre pre = Pre()) {
-  finish_long_rows_geo<kTail, V, NR>(P, epi, acc, lds, pre);
+  return finish_long_rows_geo<kTail, V, NR>(P, epi, acc, lds, pre);
+}
+// The iteration tails of the single-LP engine: inputs staged by the product kernel.
+template <typename V, int NR, int NS, int NM, class E, class Pre>
+__device__ __forceinline__ bool finish_staged_rows(const PlanView<V>& P, E& epi, Acc<NS, NM>& acc,
+                                                   double* lds, Pre pre) {
+  return finish_long_rows_geo<kTail, V, NR, true>(P, epi, acc, lds, pre);
 }
 
 // Adds p[q * stride + k] for q in [0, count) into out[k], k < NV, with the loads of several
@@ -540,6 +669,34 @@ __device__ __forceinline__ void lane_sums_1_2(const double* a, int64_t ca, const
       sb[1] += vb[u][1];
     }
   }
+}
+
+// lane_sums_1_2 over arrays padded with zeros to whole rounds of kSumU * blockDim entries: a
+// holds `rounds` such rounds and b twice that, rounds >= 1. Every load is unconditional, and
+// the first round is issued before any loop, beside whatever the caller issued before: no load
+// waits for another. Same order of additions as lane_sums_1_2, so the same sums bit for bit.
+constexpr int kSumU = 4;
+__device__ __forceinline__ void lane_sums_1_2_padded(const double* a, const double* b,
+                                                     int64_t rounds, double (&sa)[1],
+                                                     double (&sb)[2]) {
+  auto round = [&](int64_t r) {
+    double va[kSumU];
+    double2 vb[kSumU];
+#pragma unroll
+    for (int u = 0; u < kSumU; ++u) {
+      const int64_t q = (r * kSumU + u) * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+      va[u] = a[q];
+      vb[u] = reinterpret_cast<const double2*>(b)[q];
+    }
+#pragma unroll
+    for (int u = 0; u < kSumU; ++u) {
+      sa[0] += va[u];
+      sb[0] += vb[u].x;
+      sb[1] += vb[u].y;
+    }
+  };
+  round(0);
+  for (int64_t r = 1; r < rounds; ++r) round(r);
 }
 
 }  // namespace sankhya_hip

@@ -159,10 +159,14 @@ def test_device_memory_estimate_against_a_hand_count():
     csr = sh.host_csr(mdl)
     m, n, nnz = 2, 3, 4
     ctl, tick = 304, 48  # sizeof(Ctl), sizeof(pdhg_tick)
+    # row pointers, indices, values, one item, no long rows / tail groups / segment sums /
+    # staged long-row inputs
     plan = lambda rows: (_bytes(rows + 1, 4) + _bytes(nnz, 4) + _bytes(nnz, 8) + _bytes(1, 16)
-                         + _bytes(0, 16) + _bytes(0, 16) + _bytes(0, 8))
+                         + _bytes(0, 16) + _bytes(0, 16) + _bytes(0, 8) + _bytes(0, 8))
     want = (plan(m) + plan(n) + _bytes(2 * n, 8) + _bytes(2 * m, 8) + 6 * _bytes(n, 8) + 6 * _bytes(m, 8)
-            + 7 * _bytes(n, 8) + 5 * _bytes(m, 8) + _bytes(1, 8) + _bytes(1, 8) + _bytes(2, 8)
+            # partials of k_primal and k_dual padded to one round of the tail's loads (4 x 1024),
+            # the long-column and long-row sums, the tick partials
+            + 7 * _bytes(n, 8) + 5 * _bytes(m, 8) + _bytes(4096, 8) + _bytes(1, 8) + _bytes(8192, 8)
             + _bytes(1, 8) + _bytes(9, 8) + _bytes(9, 8) + _bytes(1, ctl) + _bytes(1 << 16, tick)
             + _bytes(2, 8)
             + _bytes(2 * 1024 + 2, 8))
