@@ -49,7 +49,8 @@ constexpr int kDefaultChunk = 48;
 
 // ---- iteration kernels ------------------------------------------------------------------
 
-template <typename V, typename X>
+// NT: the matrix loads' flavour (non-temporal, or cached when the LP fits the MALL).
+template <typename V, typename X, bool NT>
 __global__ __launch_bounds__(kThreads) void k_primal(PlanView<V> T, Iter<X> it, Ctl* ctl,
                                                      double* part, int force) {
   __shared__ double lds[kTile];
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(kThreads) void k_primal(PlanView<V> T, Iter<X> it, 
   PrimalEpi<X> epi = primal_epi(it, c);
   Acc<1, 0> acc;
   acc.zero();
-  run_item<V, 1>(T, item, gather, epi, acc, lds, red);
+  run_item<V, 1, NT>(T, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = acc.s[0];
 }
@@ -87,7 +88,7 @@ __global__ __launch_bounds__(kTail) void k_primal_tail(PlanView<V> T, Iter<X> it
 
 // stepf (2 doubles): the step rule's factors for this attempt, evaluated by workgroup 0 while the
 // others work, and read by k_dual_tail.
-template <typename V, typename X>
+template <typename V, typename X, bool NT>
 __global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ctl* ctl,
                                                    double* part, double* stepf, int force) {
   __shared__ double lds[kTile];
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ct
   DualEpi<X> epi = dual_epi(it, c);
   Acc<2, 0> acc;
   acc.zero();
-  run_item<V, 2>(A, item, gather, epi, acc, lds, red);
+  run_item<V, 2, NT>(A, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = acc.s[0];
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(kTail) void k_dual_tail(PlanView<V> A, Iter<X> it, 
   }
 }
 
-template <typename V, typename X>
+template <typename V, typename X, bool NT>
 __global__ __launch_bounds__(kThreads) void k_eval_rows(PlanView<V> A, Iter<X> it, Ctl* ctl,
                                                         double* part) {
   __shared__ double lds[kTile];
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_rows(PlanView<V> A, Iter<X> i
   TickRowsEpi<X> epi{it, cur, tp, count};
   Acc<7, 2> acc;
   acc.zero();
-  run_item<V, 2>(A, item, gather, epi, acc, lds, red);
+  run_item<V, 2, NT>(A, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) {
     for (int k = 0; k < 7; ++k) part[9 * blockIdx.x + k] = acc.s[k];
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_rows(PlanView<V> A, Iter<X> i
   }
 }
 
-template <typename V, typename X>
+template <typename V, typename X, bool NT>
 __global__ __launch_bounds__(kThreads) void k_eval_cols(PlanView<V> T, Iter<X> it, Ctl* ctl,
                                                         double* part) {
   __shared__ double lds[kTile];
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_cols(PlanView<V> T, Iter<X> i
   TickColsEpi<X> epi{it, cur, tp, count};
   Acc<7, 2> acc;
   acc.zero();
-  run_item<V, 2>(T, item, gather, epi, acc, lds, red);
+  run_item<V, 2, NT>(T, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) {
     for (int k = 0; k < 7; ++k) part[9 * blockIdx.x + k] = acc.s[k];
@@ -220,8 +221,9 @@ __global__ __launch_bounds__(kTail) void k_decide(PlanView<V> A, PlanView<V> T, 
   Acc<7, 2> ra, ca;
   ra.zero();
   ca.zero();
-  finish_long_rows<V, 2>(A, repi, ra, lds);
-  finish_long_rows<V, 2>(T, cepi, ca, lds);
+  // The long rows' and long columns' inputs were staged by k_eval_rows and k_eval_cols.
+  finish_staged_rows<V, 2>(A, repi, ra, lds, NoPre());
+  finish_staged_rows<V, 2>(T, cepi, ca, lds, NoPre());
   block_reduce(ra, red);
   block_reduce(ca, red);
   constexpr bool kMax9[9] = {false, false, false, false, false, false, false, true, true};
@@ -584,7 +586,9 @@ struct DevPlan {
     p.has_group0 = ngroups > 0 ? 1 : 0;
     return p;
   }
-  // The view of an iteration kernel and its tail, which stage the long rows' epilogue inputs.
+  // The view of a product kernel and its tail (an attempt's, or the tick's) that pass the long
+  // rows' epilogue inputs through lr_in. The pairs run one after another on one stream, each
+  // pair's staging read before the next pair writes.
   template <typename V>
   PlanView<V> staged_view() const {
     PlanView<V> p = view<V>();
@@ -652,6 +656,10 @@ struct pdhg_dev {
   hipGraphExec_t exec = nullptr;
   int exec_arith = -1;  // precision the captured graph was built for
   int chunk = kDefaultChunk;
+  // Matrix loads of the iteration kernels cached rather than non-temporal: the handle's device
+  // arrays fit the MALL with room to spare (kMallResident), so the matrices stay there between
+  // attempts.
+  bool cached = false;
   bool uploaded = false, started = false;
   int64_t bytes = 0;
   std::vector<void*> allocs;
@@ -782,6 +790,21 @@ Iter<X> iter_of(const pdhg_dev* h) {
   return it;
 }
 
+// What an attempt reads: both matrices in the iteration precision (row pointers, indices,
+// values), the iterate pairs, and per column the cost, bounds and running sum, per row the
+// bounds and running sum (the restart point and anchors in the FP32 phase).
+int64_t attempt_bytes(const pdhg_dev* h) {
+  const int64_t vb = h->precision == PDHG_PRECISION_MIXED ? 4 : 8;
+  const int64_t m = h->m, n = h->n;
+  return 2 * h->nnz * (4 + vb) + 4 * (m + n + 2) + n * (2 * vb + 4 * 8) + m * (2 * vb + 3 * 8) +
+         (vb == 4 ? 16 * (m + n) : 0);
+}
+// The MI355X's MALL (Infinity Cache) holds 256 MB; an LP whose attempt reads at most this much
+// runs its iteration kernels with cached matrix loads (the matrices stay resident between
+// attempts), a larger one with non-temporal loads. The margin leaves room for the tick's and
+// the report's arrays.
+constexpr int64_t kMallResident = int64_t{200} << 20;
+
 // Rounds of k_dual_tail's unconditional partial-sum loads (lane_sums_1_2_padded): the buffers
 // part_p (one partial per item of Ahat^T) and part_d (two per item of Ahat) hold that many
 // rounds of kSumU * kTail entries, zero past the items.
@@ -796,18 +819,36 @@ int apply_grid(const pdhg_dev* h) {
   return static_cast<int>(std::min<int64_t>((w + kThreads - 1) / kThreads, 8192));
 }
 
+// The two iteration kernels with the handle's matrix-load flavour (pdhg_dev::cached).
+template <typename V, typename X>
+void launch_primal(pdhg_dev* h, int force) {
+  const Iter<X> it = iter_of<X>(h);
+  if (h->cached)
+    k_primal<V, X, false><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.staged_view<V>(), it,
+                                                                   h->ctl, h->part_p, force);
+  else
+    k_primal<V, X, true><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.staged_view<V>(), it,
+                                                                  h->ctl, h->part_p, force);
+}
+template <typename V, typename X>
+void launch_dual(pdhg_dev* h, int force) {
+  const Iter<X> it = iter_of<X>(h);
+  if (h->cached)
+    k_dual<V, X, false><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.staged_view<V>(), it, h->ctl,
+                                                                 h->part_d, h->stepf, force);
+  else
+    k_dual<V, X, true><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.staged_view<V>(), it, h->ctl,
+                                                                h->part_d, h->stepf, force);
+}
+
 template <typename V, typename X>
 void launch_attempt(pdhg_dev* h, int force) {
   const Iter<X> it = iter_of<X>(h);
-  if (h->T.nitems > 0)
-    k_primal<V, X><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.staged_view<V>(), it, h->ctl,
-                                                            h->part_p, force);
+  if (h->T.nitems > 0) launch_primal<V, X>(h, force);
   if (h->T.nlr > 0)
     k_primal_tail<V, X><<<1, kTail, 0, h->stream>>>(h->T.staged_view<V>(), it, h->ctl, h->lr_p,
                                                     force);
-  if (h->A.nitems > 0)
-    k_dual<V, X><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.staged_view<V>(), it, h->ctl,
-                                                          h->part_d, h->stepf, force);
+  if (h->A.nitems > 0) launch_dual<V, X>(h, force);
   k_dual_tail<V, X><<<1, kTail, 0, h->stream>>>(h->A.staged_view<V>(), it, h->ctl, h->part_p,
                                                 h->lr_p, h->T.nlr > 0 ? 1 : 0, h->part_d,
                                                 h->A.nitems, sum_rounds(h->A.nitems, h->T.nitems),
@@ -834,15 +875,28 @@ void launch_chunk(pdhg_dev* h) {
   for (int s = 0; s < h->chunk; ++s) launch_attempt<V, X>(h, 0);
   // The evaluation tick reads the FP64 values in either precision: every convergence,
   // restart and primal-weight decision is taken on FP64 residuals.
-  if (h->A.nitems > 0)
-    k_eval_rows<double, X><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.view<double>(), it, h->ctl,
-                                                                    h->part_er);
-  if (h->T.nitems > 0)
-    k_eval_cols<double, X><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.view<double>(), it, h->ctl,
-                                                                    h->part_ec);
-  k_decide<double, X><<<1, kTail, 0, h->stream>>>(h->A.view<double>(), h->T.view<double>(), it,
-                                                  h->ctl, h->part_er, h->A.nitems, h->part_ec,
-                                                  h->T.nitems, h->ticks);
+  // Cached FP64 matrix loads when the FP64 matrices are the iteration's (the cached FP64 case).
+  const bool cached64 = h->cached && h->precision == PDHG_PRECISION_FP64;
+  if (h->A.nitems > 0) {
+    if (cached64)
+      k_eval_rows<double, X, false><<<h->A.nitems, kThreads, 0, h->stream>>>(
+          h->A.staged_view<double>(), it, h->ctl, h->part_er);
+    else
+      k_eval_rows<double, X, true><<<h->A.nitems, kThreads, 0, h->stream>>>(
+          h->A.staged_view<double>(), it, h->ctl, h->part_er);
+  }
+  if (h->T.nitems > 0) {
+    if (cached64)
+      k_eval_cols<double, X, false><<<h->T.nitems, kThreads, 0, h->stream>>>(
+          h->T.staged_view<double>(), it, h->ctl, h->part_ec);
+    else
+      k_eval_cols<double, X, true><<<h->T.nitems, kThreads, 0, h->stream>>>(
+          h->T.staged_view<double>(), it, h->ctl, h->part_ec);
+  }
+  k_decide<double, X><<<1, kTail, 0, h->stream>>>(h->A.staged_view<double>(),
+                                                  h->T.staged_view<double>(), it, h->ctl,
+                                                  h->part_er, h->A.nitems, h->part_ec, h->T.nitems,
+                                                  h->ticks);
   k_apply<X><<<apply_grid(h), kThreads, 0, h->stream>>>(it, h->ctl);
   if constexpr (kDelta<X>) launch_anchor(h, 0);
   k_clear<<<1, 64, 0, h->stream>>>(h->ctl);
@@ -1171,6 +1225,7 @@ int pdhg_dev_upload(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz, const int32_
   c.halt = 1;
   HIP_TRY(h, hipMemcpyAsync(h->ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->cached = attempt_bytes(h) <= kMallResident;
   h->uploaded = true;
   return PDHG_OK;
 }
@@ -1502,17 +1557,15 @@ int pdhg_dev_time_kernels(pdhg_dev* h, int reps, double* primal_ms, double* dual
   HIP_TRY(h, hipEventCreate(&e0));
   HIP_TRY(h, hipEventCreate(&e1));
   HIP_TRY(h, hipEventCreate(&e2));
-  const Iter<double> itd = iter_of<double>(h);
-  const Iter<float> itf = iter_of<float>(h);
   HIP_TRY(h, hipEventRecord(e0, h->stream));
   for (int r = 0; r < reps && h->T.nitems > 0; ++r) {
-    if (mixed(h)) k_primal<float, float><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.staged_view<float>(), itf, h->ctl, h->part_p, 1);
-    else k_primal<double, double><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.staged_view<double>(), itd, h->ctl, h->part_p, 1);
+    if (mixed(h)) launch_primal<float, float>(h, 1);
+    else launch_primal<double, double>(h, 1);
   }
   HIP_TRY(h, hipEventRecord(e1, h->stream));
   for (int r = 0; r < reps && h->A.nitems > 0; ++r) {
-    if (mixed(h)) k_dual<float, float><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.staged_view<float>(), itf, h->ctl, h->part_d, h->stepf, 1);
-    else k_dual<double, double><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.staged_view<double>(), itd, h->ctl, h->part_d, h->stepf, 1);
+    if (mixed(h)) launch_dual<float, float>(h, 1);
+    else launch_dual<double, double>(h, 1);
   }
   HIP_TRY(h, hipEventRecord(e2, h->stream));
   HIP_TRY(h, hipEventSynchronize(e2));

@@ -546,8 +546,12 @@ __device__ inline void step_decide(Ctl* ctl, const StepSnap& sn, double mx, doub
 // Best tracking, stopping test, restart and primal weight (src/pdhg/pdhg.cpp:344-483) from a
 // tick's 9-wide row and column totals, in one thread. Records the tick in `ticks` (a ring of
 // kTickCap records) when that is not null.
-__device__ inline void tick_decide(Ctl* ctl, const double (&vr)[9], const double (&vc)[9],
+__device__ inline void tick_decide(Ctl* ctl_mem, const double (&vr)[9], const double (&vc)[9],
                                    pdhg_tick* ticks) {
+  // On a copy in registers: every field is loaded at once and stored once at the end. Worked
+  // in place, each read of a field after a store to the block is its own memory round trip.
+  Ctl local = *ctl_mem;
+  Ctl* const ctl = &local;
   // rows: 0 pv_c 1 sup_c 2 dv_c 3 pv_a 4 sup_a 5 dv_a 6 dyr 7 cm_c 8 cm_a
   // cols: 0 dv_c 1 bc_c 2 po_c 3 dv_a 4 bc_a 5 po_a 6 dxr 7 cm_c 8 cm_a
   const double bn = ctl->bound_norm, cn = ctl->cost_norm;
@@ -627,6 +631,7 @@ __device__ inline void tick_decide(Ctl* ctl, const double (&vr)[9], const double
   } else {
     ctl->halt = 0;
   }
+  *ctl_mem = local;
 }
 
 // The tick's apply step for one column j / one row i (src/pdhg/pdhg.cpp:401-420,471-482): copy

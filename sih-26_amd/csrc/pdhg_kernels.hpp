@@ -55,24 +55,37 @@ __device__ __forceinline__ double project(double v, double lo, double hi) {
   return v;
 }
 
-// The matrix is streamed once per product: load it non-temporally so it does not evict the
-// gathered vectors from L2 and the MALL.
-template <typename T>
-__device__ __forceinline__ T stream_load(const T* p) {
-  return __builtin_nontemporal_load(p);
+// The matrix is streamed once per product. NT = true loads it non-temporally, so that it does
+// not evict the gathered vectors from L2 and the 256 MB MALL (the Infinity Cache); that pays on
+// an LP whose matrices do not fit the MALL. When the whole working set fits (NT = false), plain
+// loads leave the matrix in the MALL from one product to the next, and the next attempt reads
+// it from there: on the 200,000 x 500,000 Zipf LP (about 190 MB on the device) an iteration
+// took 58.6 us with plain loads against 62.4 us non-temporal; on the 1,000,000 x 2,000,000 LP
+// (about 750 MB) 229 us against 213 us.
+template <bool NT, typename T>
+__device__ __forceinline__ T mat_load(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
 }
+template <bool NT = true, typename T>
+__device__ __forceinline__ T stream_load(const T* p) {
+  return mat_load<NT>(p);
+}
+template <bool NT = true>
 __device__ __forceinline__ int2 stream_load(const int2* p) {
-  const long long v = __builtin_nontemporal_load(reinterpret_cast<const long long*>(p));
+  const long long v = mat_load<NT>(reinterpret_cast<const long long*>(p));
   return make_int2(static_cast<int>(v & 0xffffffffLL), static_cast<int>(v >> 32));
 }
 // Two consecutive matrix values (the pointer is pair-aligned), widened to double.
+template <bool NT = true>
 __device__ __forceinline__ double2 load_vals2(const double* p) {
   typedef double d2 __attribute__((ext_vector_type(2)));
-  const d2 v = __builtin_nontemporal_load(reinterpret_cast<const d2*>(p));
+  const d2 v = mat_load<NT>(reinterpret_cast<const d2*>(p));
   return make_double2(v.x, v.y);
 }
+template <bool NT = true>
 __device__ __forceinline__ double2 load_vals2(const float* p) {
-  const double v = __builtin_nontemporal_load(reinterpret_cast<const double*>(p));
+  const double v = mat_load<NT>(reinterpret_cast<const double*>(p));
   float2 f;
   __builtin_memcpy(&f, &v, sizeof(f));
   return make_double2(static_cast<double>(f.x), static_cast<double>(f.y));
@@ -324,7 +337,7 @@ using SingleGeo = Geometry<kThreads, kTile, kSeg, kMaxRows>;
 // Memory round trips: the item descriptor arrives with the caller's control-block loads; then
 // ONE round trip brings the item's nonzeros, its row extents and every epilogue input; then the
 // gathers. Products are staged in LDS and each row is summed by L lanes with shuffles.
-template <class GEO, typename V, int NR, int NS, int NM, class G, class E>
+template <class GEO, typename V, int NR, bool NT = true, int NS, int NM, class G, class E>
 __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& epi,
                              Acc<NS, NM>& acc, double* lds, double* red) {
   constexpr int kThreads = GEO::kThreads, kTile = GEO::kTile, kSeg = GEO::kSeg;
@@ -346,8 +359,8 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
     for (int u = 0; u < U; ++u) {
       const int k = kb + 2 * (u * kThreads + tid);
       if (k < k1) {
-        cidx[u] = stream_load(reinterpret_cast<const int2*>(P.col + k));
-        aval[u] = load_vals2(P.val + k);
+        cidx[u] = stream_load<NT>(reinterpret_cast<const int2*>(P.col + k));
+        aval[u] = load_vals2<NT>(P.val + k);
       }
     }
     int rs[kRowsPerLane], re[kRowsPerLane];
@@ -457,8 +470,8 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
     for (int u = 0; u < U; ++u) {
       const int k = kb + 2 * (u * kThreads + tid);
       if (k < k1) {
-        cidx[u] = stream_load(reinterpret_cast<const int2*>(P.col + k));
-        aval[u] = load_vals2(P.val + k);
+        cidx[u] = stream_load<NT>(reinterpret_cast<const int2*>(P.col + k));
+        aval[u] = load_vals2<NT>(P.val + k);
       }
     }
     // Gathers in batches of kBatch steps, each batch issued whole before its products are
@@ -509,10 +522,10 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
   }
 }
 
-template <typename V, int NR, int NS, int NM, class G, class E>
+template <typename V, int NR, bool NT = true, int NS, int NM, class G, class E>
 __device__ __forceinline__ void run_item(const PlanView<V>& P, const int4 it, G& gather, E& epi,
                                          Acc<NS, NM>& acc, double* lds, double* red) {
-  run_item_geo<SingleGeo, V, NR>(P, it, gather, epi, acc, lds, red);
+  run_item_geo<SingleGeo, V, NR, NT>(P, it, gather, epi, acc, lds, red);
 }
 
 // Completes every long row of P in one workgroup (launched after the kernel that wrote the
