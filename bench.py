@@ -22,7 +22,7 @@ Other fields of the line:
   - `resident`: the same solves of the LP held in HBM (uploaded once; each solve preconditions
     again on the device from the kept values), and whether they end on the same iteration and
     point bit for bit as the whole solve();
-  - `reference_rule`: the 1,000 x 3,000 uniform-equality LPs (seeds 1-4) solved under the
+  - `reference_rule`: the 1,000 x 3,000 uniform-equality LPs (seeds 1 and 2) solved under the
     reference's own stopping test at 1e-6 (pdhg_stop_at_request), GPU whole solves, beside the
     CPU oracle's full solve of seed 1 on one core and on all cores (A^T y gathered on every
     core, as the reference's threads option does);
@@ -34,9 +34,9 @@ Other fields of the line:
     nonzeros per column) solved in one batch in mixed precision to relative KKT 1e-6, and again
     polished to 1e-8 in FP64 (pdhg_batch_promote), each against plain FP64 single-LP solves of a
     sample of the same LPs at the same tolerance; `speedup` is the ratio of the LPs/s;
-  - `adjoint`: 64 blend-style LPs (m = 5,000, n = 12,000) solved toward relative KKT 2e-10, then
-    the adjoint: forward and backward seconds, and the envelope error of L = c.x* (dL/db
-    against y*) over the LPs with a square active system;
+  - `adjoint`: 64 blend-style LPs (m = 5,000, n = 12,000) solved toward relative KKT 2e-10 within
+    1,000,000 iterations, then the adjoint: forward and backward seconds, and the envelope error
+    of L = c.x* (dL/db against y*) over the LPs with a square active system;
   - `roofline`: the dominant kernel's algorithmic bytes over its HIP-event-timed duration.
 
 Prints one JSON line. One GPU only: --gpus above 1 exits with a message.
@@ -157,7 +157,7 @@ def _single_sample(sh, models, opts, tol):
     return ok, ts
 
 
-def batched_comparison(sh, count, polished_sample=4, polished_lps=128, polished_cap=500_000):
+def batched_comparison(sh, count, polished_sample=4, polished_lps=64, polished_cap=500_000):
     """Mixed-precision batched solves of `count` LPs of the batched workload to relative KKT
     1e-6, against FP64 single-LP solves of a 16-LP sample of the same LPs (every count/16-th),
     whose LPs/s stand for the whole workload. Then the FP64-polished form: the first
@@ -229,7 +229,7 @@ def batched_comparison(sh, count, polished_sample=4, polished_lps=128, polished_
     return out
 
 
-def reference_rule_check(sh, seeds=(1, 2, 3, 4), cpu=True):
+def reference_rule_check(sh, seeds=(1, 2), cpu=True):
     """The reference's own definition of solved (meets_request at 1e-6 with
     pdhg_stop_at_request: relative primal, dual, gap <= 1e-6 and absolute primal <= 1e-7) on the
     1,000 x 3,000 uniform-equality LPs: GPU whole solves of each seed, and the CPU oracle's full
@@ -278,7 +278,7 @@ def reference_rule_check(sh, seeds=(1, 2, 3, 4), cpu=True):
     return out
 
 
-def c2_check(sh, iteration_cap=100_000, reps=20):
+def c2_check(sh, iteration_cap=30_000, reps=20):
     """The 1,000,000 x 2,000,000 Zipf LP (nnz 20M, seed 3): mixed precision, FP32 phase to
     relative KKT 1e-6, then the FP64 polish toward 1e-8 under an iteration cap; and a plain FP64
     solve to 1e-6 whose objective the mixed result is compared with."""
@@ -327,9 +327,9 @@ def c2_check(sh, iteration_cap=100_000, reps=20):
                 "objective_rel_diff_vs_fp64_1e6": rel(c), "status": c.status}}
 
 
-def adjoint_check(sh, count, tolerance=2e-10, floor=1e-9):
+def adjoint_check(sh, count, tolerance=2e-10, floor=1e-9, iteration_limit=1_000_000):
     """The differentiable-LP workload: `count` blend LPs (m = 5,000, n = 12,000, seeds 5000+i)
-    solved in one batch asking relative KKT <= `tolerance` (iteration limit 2,000,000), then the
+    solved in one batch asking relative KKT <= `tolerance` (within `iteration_limit`), then the
     adjoint (the backward of the torch layer) for L = c.x* and for L = w.x* (w ~ N(0,1)).
     Envelope check of the first, dL/db against the row duals y*, relative to the largest entry
     (and, beside it, in the 2-norm), on every LP whose forward ended at relative KKT <= `floor`
@@ -350,7 +350,7 @@ def adjoint_check(sh, count, tolerance=2e-10, floor=1e-9):
     log(f"[bench] adjoint workload: {count} blend LPs (generated in {time.perf_counter()-t:.1f}s)")
     t = time.perf_counter()
     ses = sh.BatchSession(models, pdhg_tolerance=tolerance, pdhg_stop_rule="relative_kkt",
-                          iteration_limit=2_000_000, pdhg_polish=False)
+                          iteration_limit=iteration_limit, pdhg_polish=False)
     tf = time.perf_counter() - t
     sols = ses.solutions
     solved = [s.reached(floor) for s in sols]
@@ -391,7 +391,7 @@ def adjoint_check(sh, count, tolerance=2e-10, floor=1e-9):
     log(f"[bench] adjoint backward: c.x* {te:.2f}s, w.x* {tw:.2f}s; envelope error on {square} "
         f"square LPs {env_b:.1e}; {degenerate} degenerate (active-system residual {deg_resid:.1e})")
     return {"workload": f"blend-{count}: m = 5000, n = 12000, 40% of columns upper-bounded, "
-                        f"forward asked relative KKT <= {tolerance:g} within 2,000,000 "
+                        f"forward asked relative KKT <= {tolerance:g} within {iteration_limit:,} "
                         f"iterations, envelope checked where it ended <= {floor:g}; FP64 adjoint",
             "lps": count, "reached_tolerance": reached, "solved": int(sum(solved)),
             "forward_seconds": tf,

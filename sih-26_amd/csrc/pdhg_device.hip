@@ -208,9 +208,8 @@ __global__ __launch_bounds__(kThreads) void k_eval_cols(PlanView<V> T, Iter<X> i
 // completing the long rows of both products.
 template <typename V, typename X>
 __global__ __launch_bounds__(kTail) void k_decide(PlanView<V> A, PlanView<V> T, Iter<X> it,
-                                                  Ctl* ctl, const double* per, int nper,
-                                                  const double* pec, int npec,
-                                                  pdhg_tick* ticks) {
+                                                  Ctl* ctl, const double* per, const double* pec,
+                                                  int rounds, pdhg_tick* ticks) {
   __shared__ double lds[2 * kChunk];
   __shared__ double red[16 * 9];
   if (!ctl->tick) return;
@@ -228,8 +227,8 @@ __global__ __launch_bounds__(kTail) void k_decide(PlanView<V> A, PlanView<V> T, 
   block_reduce(ca, red);
   constexpr bool kMax9[9] = {false, false, false, false, false, false, false, true, true};
   double vr[9], vc[9];
-  block_sums<9>(per, nper, nullptr, 0, 9, kMax9, vr, red);
-  block_sums<9>(pec, npec, nullptr, 0, 9, kMax9, vc, red);
+  // part_er and part_ec hold `rounds` whole rounds of partials, zero past the items.
+  block_sums2_padded<9>(per, pec, rounds, 9, kMax9, vr, vc, red);
   if (threadIdx.x != 0) return;
   merge9(vr, ra);
   merge9(vc, ca);
@@ -895,8 +894,8 @@ void launch_chunk(pdhg_dev* h) {
   }
   k_decide<double, X><<<1, kTail, 0, h->stream>>>(h->A.staged_view<double>(),
                                                   h->T.staged_view<double>(), it, h->ctl,
-                                                  h->part_er, h->A.nitems, h->part_ec, h->T.nitems,
-                                                  h->ticks);
+                                                  h->part_er, h->part_ec,
+                                                  sum_rounds(h->A.nitems, h->T.nitems), h->ticks);
   k_apply<X><<<apply_grid(h), kThreads, 0, h->stream>>>(it, h->ctl);
   if constexpr (kDelta<X>) launch_anchor(h, 0);
   k_clear<<<1, 64, 0, h->stream>>>(h->ctl);
@@ -1131,8 +1130,8 @@ int64_t pdhg_dev_estimate_bytes(int32_t m, int32_t n, int64_t nnz, const int32_t
   total += 7 * bytes(n, 8) + 5 * bytes(m, 8);                         // problem vectors
   const int64_t ia = std::max(items_a, 1), it = std::max(items_t, 1);
   const int64_t padded = int64_t{sum_rounds(items_a, items_t)} * kSumU * kTail;
-  total += bytes(padded, 8) + bytes(1, 8) + bytes(2 * padded, 8) + bytes(1, 8) + bytes(9 * ia, 8) +
-           bytes(9 * it, 8);
+  total += bytes(padded, 8) + bytes(1, 8) + bytes(2 * padded, 8) + bytes(1, 8) +
+           bytes(9 * padded, 8) + bytes(9 * padded, 8);
   total += bytes(1, sizeof(Ctl)) + bytes(kTickCap, sizeof(pdhg_tick)) + bytes(2, 8);
   if (unscaled) total += bytes(2 * 1024 + 2, 8);  // preconditioner scratch
   return total;
@@ -1211,13 +1210,15 @@ int pdhg_dev_upload(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz, const int32_
   const size_t padded = static_cast<size_t>(sum_rounds(h->A.nitems, h->T.nitems)) * kSumU * kTail;
   if ((rc = dalloc(h, &h->part_p, padded)) || (rc = dalloc(h, &h->lr_p, 1)) ||
       (rc = dalloc(h, &h->part_d, 2 * padded)) || (rc = dalloc(h, &h->lr_sum, 1)) ||
-      (rc = dalloc(h, &h->part_er, 9 * ia)) || (rc = dalloc(h, &h->part_ec, 9 * it)) ||
+      (rc = dalloc(h, &h->part_er, 9 * padded)) || (rc = dalloc(h, &h->part_ec, 9 * padded)) ||
       (rc = dalloc(h, &h->ctl, 1)) || (rc = dalloc(h, &h->ticks, static_cast<size_t>(kTickCap))) ||
       (rc = dalloc(h, &h->stepf, 2)))
     return rc;
-  // The padding past the items must read zero (k_dual_tail adds it).
+  // The padding past the items must read zero (k_dual_tail and k_decide add it).
   HIP_TRY(h, hipMemsetAsync(h->part_p, 0, padded * sizeof(double), h->stream));
   HIP_TRY(h, hipMemsetAsync(h->part_d, 0, 2 * padded * sizeof(double), h->stream));
+  HIP_TRY(h, hipMemsetAsync(h->part_er, 0, 9 * padded * sizeof(double), h->stream));
+  HIP_TRY(h, hipMemsetAsync(h->part_ec, 0, 9 * padded * sizeof(double), h->stream));
   Ctl c;
   std::memset(&c, 0, sizeof(c));
   c.bound_norm = bound_norm;

@@ -262,6 +262,72 @@ __device__ inline void block_sums(const double* a, int64_t ca, const double* b, 
   __syncthreads();
 }
 
+// NV sums (or maxima, where is_max[k]) of two arrays at once, each holding rounds * kSumU *
+// blockDim entries NV wide at the given stride, zero past the items: every load unconditional,
+// the loads of both arrays issued together (one memory round trip per round), then the
+// reductions. Per lane the entries are taken in index order, as block_sums takes them. The
+// totals are returned in wave 0 (the caller reads them in thread 0).
+template <int NV>
+__device__ inline void block_sums2_padded(const double* a, const double* b, int64_t rounds,
+                                          int stride, const bool (&is_max)[NV],
+                                          double (&out_a)[NV], double (&out_b)[NV],
+                                          double* red /* >= waves*NV */) {
+  constexpr int U = 4;
+  const int nwaves = static_cast<int>(blockDim.x) >> 6;
+  double ta[NV], tb[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) ta[k] = tb[k] = 0.0;
+  for (int64_t r = 0; r < rounds; ++r) {
+    double va[U][NV], vb[U][NV];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t q = (r * U + u) * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        va[u][k] = a[q * stride + k];
+        vb[u][k] = b[q * stride + k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        ta[k] = is_max[k] ? smax(ta[k], va[u][k]) : ta[k] + va[u][k];
+        tb[k] = is_max[k] ? smax(tb[k], vb[u][k]) : tb[k] + vb[u][k];
+      }
+    }
+  }
+  // Wave results through LDS; lane k of wave 0 combines value k over the waves in wave order,
+  // and the totals are read from those lanes: they are valid in wave 0 (thread 0 uses them).
+  const int lane = threadIdx.x & 63;
+  auto finish = [&](double (&t)[NV], double (&out)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) t[k] = is_max[k] ? wave_max(t[k]) : wave_sum(t[k]);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) red[(threadIdx.x >> 6) * NV + k] = t[k];
+    }
+    __syncthreads();
+    if ((threadIdx.x >> 6) == 0) {
+      double v = 0.0;
+      if (lane < NV) {
+        bool mx = false;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) mx = (lane == k) ? is_max[k] : mx;
+        v = red[lane];
+#pragma unroll 4
+        for (int w = 1; w < nwaves; ++w) v = mx ? smax(v, red[w * NV + lane]) : v + red[w * NV + lane];
+      }
+#pragma unroll
+      for (int k = 0; k < NV; ++k) out[k] = readlane_d(v, k);
+    }
+    __syncthreads();
+  };
+  finish(ta, out_a);
+  finish(tb, out_b);
+}
+
 // A CSR matrix with its work-item plan, as kernels see it.
 template <typename V>
 struct PlanView {

@@ -165,9 +165,10 @@ def test_device_memory_estimate_against_a_hand_count():
                          + _bytes(0, 16) + _bytes(0, 16) + _bytes(0, 8) + _bytes(0, 8))
     want = (plan(m) + plan(n) + _bytes(2 * n, 8) + _bytes(2 * m, 8) + 6 * _bytes(n, 8) + 6 * _bytes(m, 8)
             # partials of k_primal and k_dual padded to one round of the tail's loads (4 x 1024),
-            # the long-column and long-row sums, the tick partials
+            # the long-column and long-row sums, the tick partials (9 per item, same padding)
             + 7 * _bytes(n, 8) + 5 * _bytes(m, 8) + _bytes(4096, 8) + _bytes(1, 8) + _bytes(8192, 8)
-            + _bytes(1, 8) + _bytes(9, 8) + _bytes(9, 8) + _bytes(1, ctl) + _bytes(1 << 16, tick)
+            + _bytes(1, 8) + _bytes(9 * 4096, 8) + _bytes(9 * 4096, 8) + _bytes(1, ctl)
+            + _bytes(1 << 16, tick)
             + _bytes(2, 8)
             + _bytes(2 * 1024 + 2, 8))
     got = L.pdhg_dev_estimate_bytes(m, n, nnz, csr["rowptr"].ctypes.data, csr["colidx"].ctypes.data,

@@ -207,6 +207,44 @@ def test_a_feasible_status_still_means_the_point_is_feasible():
         assert s.primal_infeasibility <= cm.PRIMAL_FEASIBILITY
 
 
+@pytest.mark.parametrize("tol", [1e-4, 1e-6])
+def test_a_relative_kkt_stop_claims_feasibility_only_on_the_reference_request(tol):
+    """pdhg_stop_rule=relative_kkt stops on max(relative primal, dual, gap) <= tol alone. On a
+    right-hand side of 1e5 the relative primal residual is the absolute one over 1 + 1e5, so the
+    stop comes while the absolute primal residual can still be far above the 1e-7 of the
+    reference's request (src/pdhg/pdhg_evaluate.hpp:57-60). Such a stop is not reported feasible
+    or optimal (src/pdhg/pdhg.cpp:424-436): the status is iteration_limit and the message names
+    the rule; a claimed status always comes with a feasible point."""
+    s = sh.solve(cm.model_of(cm.large_rhs_lp()), pdhg_tolerance=tol, pdhg_stop_rule="relative_kkt",
+                 iteration_limit=200000, pdhg_polish=False)
+    print(f"[relative_kkt {tol:g}] {s.status} it={s.iterations} primal_infeasibility="
+          f"{s.primal_infeasibility:.3e}: {s.message}", flush=True)
+    if s.status in ("optimal", "feasible"):
+        assert s.primal_infeasibility <= cm.PRIMAL_FEASIBILITY, (s.status, s.primal_infeasibility)
+    else:
+        assert s.status == "iteration_limit", s.status
+        if s.iterations < 200000:  # the device stopped on the rule, short of the request
+            assert "pdhg_stop_rule=relative_kkt" in s.message, s.message
+            assert "no feasibility is claimed" in s.message, s.message
+            assert s.primal_infeasibility > cm.PRIMAL_FEASIBILITY
+
+
+@pytest.mark.parametrize("m,n,seed,tol", [(2000, 5000, 3, 1e-4), (200000, 500000, 2, 1e-6)])
+def test_a_relative_kkt_stop_short_of_the_request_claims_nothing(m, n, seed, tol):
+    """The row-side Zipf LPs (|b| in the tens to hundreds on the dense rows; the second is the
+    bandwidth-scale workload): the relative KKT error reaches `tol` while the absolute primal
+    residual is still far above 1e-7, so the device stops on the rule and the report claims no
+    feasibility: iteration_limit before the limit, with the rule named in the message."""
+    s = sh.solve(sh.Model.generate("zipf", m, n, 10, seed), pdhg_tolerance=tol,
+                 pdhg_stop_rule="relative_kkt", iteration_limit=1000000, pdhg_polish=False)
+    print(f"[relative_kkt {m}x{n} {tol:g}] {s.status} it={s.iterations} kkt={s.relative_kkt:.2e} "
+          f"primal_infeasibility={s.primal_infeasibility:.3e}", flush=True)
+    assert s.status == "iteration_limit" and s.iterations < 1000000, (s.status, s.iterations)
+    assert s.relative_kkt <= tol * (1 + 1e-9)
+    assert "pdhg_stop_rule=relative_kkt" in s.message and "no feasibility is claimed" in s.message
+    assert s.primal_infeasibility > cm.PRIMAL_FEASIBILITY
+
+
 def test_stop_at_request_gives_the_cheap_answer_and_never_calls_it_optimal():
     """tests/unit/test_pdhg.cpp:340-382 on Netlib adlittle at a 1e-4 request: the default run
     reaches optimal; with pdhg_stop_at_request the run stops earlier on a point reported
