@@ -46,15 +46,21 @@ namespace sankhya_hip {
 
 constexpr int kDefaultChunk = 48;
 
+// Workgroup sizes of the two iteration kernels (tiles as in SingleGeo).
+constexpr int kPrimalThreads = 256;
+constexpr int kDualThreads = 256;
+using PrimalGeo = Geometry<kPrimalThreads, kTile, kSeg, kMaxRows>;
+using DualGeo = Geometry<kDualThreads, kTile, kSeg, kMaxRows>;
+
 
 // ---- iteration kernels ------------------------------------------------------------------
 
 // NT: the matrix loads' flavour (non-temporal, or cached when the LP fits the MALL).
 template <typename V, typename X, bool NT>
-__global__ __launch_bounds__(kThreads) void k_primal(PlanView<V> T, Iter<X> it, Ctl* ctl,
-                                                     double* part, int force) {
+__global__ __launch_bounds__(kPrimalThreads) void k_primal(PlanView<V> T, Iter<X> it, Ctl* ctl,
+                                                           double* part, int force) {
   __shared__ double lds[kTile];
-  __shared__ double red[4 * 2];
+  __shared__ double red[(kPrimalThreads / 64) * 2];
   const int4 item = T.items[blockIdx.x];
   const AttemptCtl c = read_attempt(ctl);
   if (attempt_halted(c, force, item,
@@ -64,7 +70,7 @@ __global__ __launch_bounds__(kThreads) void k_primal(PlanView<V> T, Iter<X> it, 
   PrimalEpi<X> epi = primal_epi(it, c);
   Acc<1, 0> acc;
   acc.zero();
-  run_item<V, 1, NT>(T, item, gather, epi, acc, lds, red);
+  run_item_geo<PrimalGeo, V, 1, NT>(T, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = acc.s[0];
 }
@@ -89,10 +95,10 @@ __global__ __launch_bounds__(kTail) void k_primal_tail(PlanView<V> T, Iter<X> it
 // stepf (2 doubles): the step rule's factors for this attempt, evaluated by workgroup 0 while the
 // others work, and read by k_dual_tail.
 template <typename V, typename X, bool NT>
-__global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ctl* ctl,
-                                                   double* part, double* stepf, int force) {
+__global__ __launch_bounds__(kDualThreads) void k_dual(PlanView<V> A, Iter<X> it, Ctl* ctl,
+                                                       double* part, double* stepf, int force) {
   __shared__ double lds[kTile];
-  __shared__ double red[4 * 2];
+  __shared__ double red[(kDualThreads / 64) * 2];
   const int4 item = A.items[blockIdx.x];
   const AttemptCtl c = read_attempt(ctl);
   if (attempt_halted(c, force, item,
@@ -102,7 +108,7 @@ __global__ __launch_bounds__(kThreads) void k_dual(PlanView<V> A, Iter<X> it, Ct
   DualEpi<X> epi = dual_epi(it, c);
   Acc<2, 0> acc;
   acc.zero();
-  run_item<V, 2, NT>(A, item, gather, epi, acc, lds, red);
+  run_item_geo<DualGeo, V, 2, NT>(A, item, gather, epi, acc, lds, red);
   block_reduce(acc, red);
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = acc.s[0];
@@ -823,20 +829,20 @@ template <typename V, typename X>
 void launch_primal(pdhg_dev* h, int force) {
   const Iter<X> it = iter_of<X>(h);
   if (h->cached)
-    k_primal<V, X, false><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.staged_view<V>(), it,
+    k_primal<V, X, false><<<h->T.nitems, kPrimalThreads, 0, h->stream>>>(h->T.staged_view<V>(), it,
                                                                    h->ctl, h->part_p, force);
   else
-    k_primal<V, X, true><<<h->T.nitems, kThreads, 0, h->stream>>>(h->T.staged_view<V>(), it,
+    k_primal<V, X, true><<<h->T.nitems, kPrimalThreads, 0, h->stream>>>(h->T.staged_view<V>(), it,
                                                                   h->ctl, h->part_p, force);
 }
 template <typename V, typename X>
 void launch_dual(pdhg_dev* h, int force) {
   const Iter<X> it = iter_of<X>(h);
   if (h->cached)
-    k_dual<V, X, false><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.staged_view<V>(), it, h->ctl,
+    k_dual<V, X, false><<<h->A.nitems, kDualThreads, 0, h->stream>>>(h->A.staged_view<V>(), it, h->ctl,
                                                                  h->part_d, h->stepf, force);
   else
-    k_dual<V, X, true><<<h->A.nitems, kThreads, 0, h->stream>>>(h->A.staged_view<V>(), it, h->ctl,
+    k_dual<V, X, true><<<h->A.nitems, kDualThreads, 0, h->stream>>>(h->A.staged_view<V>(), it, h->ctl,
                                                                 h->part_d, h->stepf, force);
 }
 

@@ -416,9 +416,12 @@ def test_engine_spectral_norm_matches_oracle():
 
 
 def test_repeat_solves_are_bit_identical():
+    """Two solves of a row-side Zipf LP (long rows finished by the tail kernel, many restarts)
+    follow the same path for 60,000 iterations: same attempts, same point, bit for bit."""
     mdl, _ = cm.generated("zipf", 2000, 5000, 10, 3)
-    a = sh.solve(mdl, pdhg_tolerance=1e-6, pdhg_stop_at_request=True)
-    b = sh.solve(mdl, pdhg_tolerance=1e-6, pdhg_stop_at_request=True)
+    a = sh.solve(mdl, pdhg_tolerance=1e-6, pdhg_stop_at_request=True, iteration_limit=60000)
+    b = sh.solve(mdl, pdhg_tolerance=1e-6, pdhg_stop_at_request=True, iteration_limit=60000)
+    assert a.iterations == 60000 or a.status in ("optimal", "feasible")
     assert a.iterations == b.iterations and a.stats["attempts"] == b.stats["attempts"]
     assert np.array_equal(a.col_value, b.col_value) and np.array_equal(a.row_dual, b.row_dual)
 
