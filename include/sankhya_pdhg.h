@@ -157,6 +157,13 @@ int sankhya_pdhg_quality(const sankhya_model* model, const double* col_value, co
                          const double* col_dual, double dual_bound, double* row_activity,
                          double* out);
 
+/* The same quality fields by the threaded form solve() uses after a device solve (the passes
+ * over the nonzeros on `threads` host threads, rows through the CSR of A): bit for bit the values
+ * of sankhya_pdhg_quality. No reference counterpart (a test hook). */
+int sankhya_pdhg_quality_threaded(const sankhya_model* model, const double* col_value,
+                                  const double* row_dual, const double* col_dual, double dual_bound,
+                                  int threads, double* row_activity, double* out);
+
 /* The engine's VRAM check on the host, as solve() makes it before allocating anything
  * (src/core/solve.cpp:751-771): the device-memory estimate of this model for the precision
  * (pdhg_dev_estimate_bytes) plus the reserve (pdhg_dev_vram_reserve) against free_bytes.

@@ -132,6 +132,8 @@ def load_library() -> C.CDLL:
     lib.pdhg_dev_finish.argtypes = [vp, _f64p, _f64p, _f64p, _f64p]
     lib.sankhya_pdhg_csr.argtypes = [vp, C.c_int, _i32p, _i32p, _f64p]
     lib.sankhya_pdhg_quality.argtypes = [vp, _f64p, _f64p, _f64p, C.c_double, _f64p, _f64p]
+    lib.sankhya_pdhg_quality_threaded.argtypes = [vp, _f64p, _f64p, _f64p, C.c_double, C.c_int,
+                                                  _f64p, _f64p]
     lib.sankhya_pdhg_batch_open.argtypes = [C.POINTER(vp), C.c_int32, C.c_int, C.c_int, C.POINTER(vp)]
     lib.pdhg_batch_scaling.argtypes = [vp] + [_f64p] * 8
     lib.pdhg_batch_destroy.argtypes = [vp]
@@ -464,17 +466,23 @@ def host_csr(model: Model, threads: int = 1) -> Dict[str, np.ndarray]:
     return out
 
 
-def quality(model: Model, col_value, row_dual, col_dual, dual_bound: float):
+def quality(model: Model, col_value, row_dual, col_dual, dual_bound: float, threads: int = 0):
     """The engine's Solution::recompute_quality on the host (sankhya_pdhg_quality): the row
     activity and [objective, primal_infeasibility, primal_infeasibility_scaled,
     dual_infeasibility, dual_infeasibility_scaled, complementarity_violation,
-    integrality_violation, absolute_gap, relative_gap]."""
+    integrality_violation, absolute_gap, relative_gap]. With threads > 0, the threaded form a
+    device solve's report uses (sankhya_pdhg_quality_threaded)."""
     lib = load_library()
     m, n, _ = model.dims
     act, out = np.zeros(max(m, 1)), np.zeros(9)
     f = lambda v, k: np.ascontiguousarray(v if k else np.zeros(1), np.float64)
-    rc = lib.sankhya_pdhg_quality(model._h, f(col_value, n), f(row_dual, m), f(col_dual, n),
-                                  float(dual_bound), act, out)
+    if threads > 0:
+        rc = lib.sankhya_pdhg_quality_threaded(model._h, f(col_value, n), f(row_dual, m),
+                                               f(col_dual, n), float(dual_bound), int(threads),
+                                               act, out)
+    else:
+        rc = lib.sankhya_pdhg_quality(model._h, f(col_value, n), f(row_dual, m), f(col_dual, n),
+                                      float(dual_bound), act, out)
     if rc != 0:
         raise RuntimeError("sankhya_pdhg_quality failed")
     return act[:m], out

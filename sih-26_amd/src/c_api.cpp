@@ -465,6 +465,34 @@ int sankhya_pdhg_quality(const sankhya_model* h, const double* col_value, const 
   return 0;
 }
 
+int sankhya_pdhg_quality_threaded(const sankhya_model* h, const double* col_value,
+                                  const double* row_dual, const double* col_dual, double dual_bound,
+                                  int threads, double* row_activity, double* out) {
+  if (!h || !out || threads < 1) return -1;
+  const Model& md = h->model;
+  const auto n = static_cast<std::size_t>(md.num_cols()), m = static_cast<std::size_t>(md.num_rows());
+  if ((n && (!col_value || !col_dual)) || (m && (!row_dual || !row_activity))) return -1;
+  try {
+    const hip_engine::Prepared prep = hip_engine::prepare_unscaled(md, 1);
+    Solution s;
+    s.col_value.assign(col_value, col_value + n);
+    s.row_dual.assign(row_dual, row_dual + m);
+    s.col_dual.assign(col_dual, col_dual + n);
+    s.dual_bound = dual_bound;
+    hip_engine::recompute_quality_threaded(md, prep.a_rp, prep.a_ci, prep.a_v, threads, &s);
+    if (m) std::memcpy(row_activity, s.row_activity.data(), sizeof(double) * m);
+    const double vals[9] = {s.objective,          s.primal_infeasibility,
+                            s.primal_infeasibility_scaled, s.dual_infeasibility,
+                            s.dual_infeasibility_scaled,   s.complementarity_violation,
+                            s.integrality_violation,       s.absolute_gap,
+                            s.relative_gap};
+    std::memcpy(out, vals, sizeof(vals));
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
 int sankhya_pdhg_memory_check(const sankhya_model* h, int precision, int64_t free_bytes,
                               int64_t total_bytes, int64_t* required, int64_t* reserve) {
   if (!h) return -1;

@@ -139,8 +139,168 @@ std::vector<double> power_start(const Model& model, std::int64_t random_seed) {
   return v0;
 }
 
+void recompute_quality_threaded(const Model& model, const std::vector<Index>& rp,
+                                const std::vector<Index>& ci, const std::vector<double>& cv,
+                                int threads, Solution* s) {
+  const Index n = model.num_cols(), m = model.num_rows();
+  if (static_cast<Index>(s->col_value.size()) != n) return;
+  const auto un = static_cast<std::size_t>(n), um = static_cast<std::size_t>(m);
+  const int nt = std::max(1, threads);
+  const double* x = s->col_value.data();
+  // Every maximum below starts at 0.0 and takes std::max(acc, v) = acc < v ? v : acc, so a NaN
+  // never enters it and its value does not depend on the order of the v: per-thread partial
+  // maxima, combined afterwards, give the serial pass's bits.
+  struct Maxima {
+    double pi = 0.0, pis = 0.0, iv = 0.0, di = 0.0, dis = 0.0, cv = 0.0, dn = 0.0;
+  };
+  std::vector<Maxima> part(static_cast<std::size_t>(nt));
+  const auto slot = [&](std::size_t b, std::size_t count) -> Maxima& {
+    const std::size_t chunk = (count + static_cast<std::size_t>(nt) - 1) / static_cast<std::size_t>(nt);
+    return part[chunk ? std::min(b / chunk, part.size() - 1) : 0];
+  };
+  const auto record = [](Maxima& q, double violation, double scale) {
+    if (violation <= 0.0) return;
+    q.pi = std::max(q.pi, violation);
+    q.pis = std::max(q.pis, violation / std::max(1.0, scale));
+  };
+
+  // Rows: the activity (the reference's multiply skips columns whose x is exactly zero, and adds
+  // a row's terms in ascending column order, as a row of the CSR holds them), the row scale, a
+  // maximum over the same terms, and the row's primal records; in ranges of about equal nonzero
+  // count (a few dense rows can hold most of them).
+  s->row_activity.assign(um, 0.0);
+  std::vector<double> row_scale(um, 1.0);
+  std::vector<std::size_t> cut(static_cast<std::size_t>(nt) + 1, um);
+  cut[0] = 0;
+  {
+    const std::int64_t nnz = um ? rp[um] : 0;
+    std::size_t i = 0;
+    for (int t = 1; t < nt; ++t) {
+      const std::int64_t want = nnz * t / nt;
+      while (i < um && rp[i] < want) ++i;
+      cut[static_cast<std::size_t>(t)] = i;
+    }
+  }
+  parallel_ranges(static_cast<std::size_t>(nt), nt, [&](std::size_t tb, std::size_t te) {
+    for (std::size_t t = tb; t < te; ++t) {
+      Maxima& q = part[t];
+      for (std::size_t i = cut[t]; i < cut[t + 1]; ++i) {
+        double act = 0.0, scale = 1.0;
+        for (Index k = rp[i]; k < rp[i + 1]; ++k) {
+          const double xj = x[ci[k]];
+          if (xj == 0.0) continue;
+          act += cv[k] * xj;
+          scale = std::max(scale, std::fabs(cv[k] * xj));
+        }
+        s->row_activity[i] = act;
+        row_scale[i] = scale;
+        if (is_finite_bound(model.row_lower[i])) record(q, model.row_lower[i] - act, scale);
+        if (is_finite_bound(model.row_upper[i])) record(q, act - model.row_upper[i], scale);
+      }
+    }
+  });
+  // Columns: bound violations and integrality.
+  parallel_ranges(un, nt, [&](std::size_t b, std::size_t e) {
+    Maxima& q = slot(b, un);
+    for (std::size_t j = b; j < e; ++j) {
+      const double xj = x[j];
+      if (is_finite_bound(model.col_lower[j])) record(q, model.col_lower[j] - xj, std::fabs(xj));
+      if (is_finite_bound(model.col_upper[j])) record(q, xj - model.col_upper[j], std::fabs(xj));
+      if (model.col_type[j] == VarType::kInteger) q.iv = std::max(q.iv, std::fabs(xj - std::round(xj)));
+    }
+  });
+  s->primal_infeasibility = 0.0;
+  s->primal_infeasibility_scaled = 0.0;
+  s->integrality_violation = 0.0;
+  for (const Maxima& q : part) {
+    s->primal_infeasibility = std::max(s->primal_infeasibility, q.pi);
+    s->primal_infeasibility_scaled = std::max(s->primal_infeasibility_scaled, q.pis);
+    s->integrality_violation = std::max(s->integrality_violation, q.iv);
+  }
+  s->objective = model.evaluate_objective(x);
+
+  if (s->row_dual.size() == um && s->col_dual.size() == un) {
+    const double sense = model.sense_multiplier();
+    const double* yd = s->row_dual.data();
+    parallel_ranges(um, nt, [&](std::size_t b, std::size_t e) {
+      Maxima& q = slot(b, um);
+      for (std::size_t i = b; i < e; ++i) q.dn = std::max(q.dn, std::fabs(yd[i]));
+    });
+    double dual_norm = 0.0;
+    for (const Maxima& q : part) dual_norm = std::max(dual_norm, q.dn);
+    const auto record_dual = [](Maxima& q, double violation, double scale) {
+      if (violation <= 0.0) return;
+      q.di = std::max(q.di, violation);
+      q.dis = std::max(q.dis, violation / scale);
+    };
+    const auto nearest = [](double value, double lo, double hi) {
+      double d = kInfinity;
+      if (is_finite_bound(lo)) d = std::min(d, std::fabs(value - lo));
+      if (is_finite_bound(hi)) d = std::min(d, std::fabs(value - hi));
+      return d;
+    };
+    const auto record_compl = [](Maxima& q, double mult, double slack, double mscale,
+                                 double pscale) {
+      if (mult == 0.0 || !is_finite_bound(slack)) return;
+      const double product = mult * slack;
+      q.cv = std::max(q.cv, product);
+      q.dis = std::max(q.dis, product / std::max(1.0, mscale * pscale));
+    };
+    const bool linear = !model.has_quadratic_objective() && !model.has_integrality();
+    // Columns: the implied reduced cost and term scale (the column's terms in its own order),
+    // then the column's dual and complementarity records.
+    parallel_ranges(un, nt, [&](std::size_t b, std::size_t e) {
+      Maxima& q = slot(b, un);
+      for (std::size_t j = b; j < e; ++j) {
+        double implied = model.col_cost[j];
+        double terms = std::fabs(model.col_cost[j]);
+        const ColumnView c = model.matrix.column(static_cast<Index>(j));
+        for (Index k = 0; k < c.size; ++k) {
+          const double term = c.values[k] * yd[c.rows[k]];
+          implied -= term;
+          terms = std::max(terms, std::fabs(term));
+        }
+        const double scale = std::max(1.0, terms);
+        if (linear) record_dual(q, std::fabs(implied - s->col_dual[j]), std::max(1.0, terms));
+        const double d = sense * s->col_dual[j];
+        const double xj = x[j];
+        const double lo = model.col_lower[j], hi = model.col_upper[j];
+        if (lo == hi) continue;
+        if (d > 0.0 && !is_finite_bound(lo)) record_dual(q, d, scale);
+        if (d < 0.0 && !is_finite_bound(hi)) record_dual(q, -d, scale);
+        record_compl(q, std::fabs(d), nearest(xj, lo, hi), scale, std::max(1.0, std::fabs(xj)));
+      }
+    });
+    // Rows: dual sign and complementarity records.
+    const double price_scale = std::max(1.0, dual_norm);
+    parallel_ranges(um, nt, [&](std::size_t b, std::size_t e) {
+      Maxima& q = slot(b, um);
+      for (std::size_t i = b; i < e; ++i) {
+        const double yv = sense * yd[i];
+        const double a = s->row_activity[i];
+        const double lo = model.row_lower[i], hi = model.row_upper[i];
+        if (lo == hi) continue;
+        if (yv > 0.0 && !is_finite_bound(lo)) record_dual(q, yv, price_scale);
+        if (yv < 0.0 && !is_finite_bound(hi)) record_dual(q, -yv, price_scale);
+        record_compl(q, std::fabs(yv), nearest(a, lo, hi), price_scale, row_scale[i]);
+      }
+    });
+    s->dual_infeasibility = 0.0;
+    s->dual_infeasibility_scaled = 0.0;
+    s->complementarity_violation = 0.0;
+    for (const Maxima& q : part) {
+      s->dual_infeasibility = std::max(s->dual_infeasibility, q.di);
+      s->dual_infeasibility_scaled = std::max(s->dual_infeasibility_scaled, q.dis);
+      s->complementarity_violation = std::max(s->complementarity_violation, q.cv);
+    }
+  }
+  s->absolute_gap = std::fabs(s->objective - s->dual_bound);
+  s->relative_gap = s->absolute_gap / std::max(1.0, std::fabs(s->objective));
+}
+
 void report(const Model& model, const double* res9, const std::vector<double>& y,
-            const std::vector<double>& reduced, const ReportState& rs, Solution* solution) {
+            const std::vector<double>& reduced, const ReportState& rs, Solution* solution,
+            const Prepared* prep) {
   const Residuals final_r = from_array(res9);
   const double sense = model.sense_multiplier();
   const std::size_t n = reduced.size(), m = y.size();
@@ -195,7 +355,11 @@ void report(const Model& model, const double* res9, const std::vector<double>& y
                              : LimitReason::kNone;
   solution->dual_bound = verifiable ? sense * final_r.dual_objective + model.objective_offset
                                     : (model.sense == ObjSense::kMaximize ? kInfinity : -kInfinity);
-  solution->recompute_quality(model);
+  // With the CSR of A's own values at hand (prepare_unscaled), the quality passes run threaded.
+  if (prep && !prep->a_rp.empty() && prep->scaling.row.empty())
+    recompute_quality_threaded(model, prep->a_rp, prep->a_ci, prep->a_v, host_threads(), solution);
+  else
+    solution->recompute_quality(model);
 }
 
 MemoryCheck check_device_memory(std::int64_t required, std::int64_t free_bytes,

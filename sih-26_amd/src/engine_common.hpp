@@ -67,9 +67,21 @@ struct ReportState {
   bool relative_kkt;  // the device stopped under pdhg_stop_rule=relative_kkt
 };
 // Duals with the reference's sign flip, status and message, dual bound, recompute_quality.
-// col_value must already hold the point; solve_seconds is left to the caller.
+// col_value must already hold the point; solve_seconds is left to the caller. With `prep` (from
+// prepare_unscaled: the CSR of A's own values) the quality passes run on the host's threads.
 void report(const Model& model, const double* res9, const std::vector<double>& y,
-            const std::vector<double>& reduced, const ReportState& rs, Solution* solution);
+            const std::vector<double>& reduced, const ReportState& rs, Solution* solution,
+            const Prepared* prep = nullptr);
+
+// Solution::recompute_quality (src/core/model.cpp:247-441) with its passes over the nonzeros
+// spread over `threads` host threads: the row activities and row scales walk the CSR of A's own
+// values (rp, ci, cv) by rows, each row's terms in ascending column order, the order in which
+// the reference's column-by-column scatter adds them; the implied reduced costs walk the
+// columns; the passes that take maxima over rows or columns stay serial, in the reference's
+// order. The quality fields equal the serial function's bit for bit.
+void recompute_quality_threaded(const Model& model, const std::vector<Index>& rp,
+                                const std::vector<Index>& ci, const std::vector<double>& cv,
+                                int threads, Solution* s);
 
 // The dispatcher's VRAM decision (src/core/solve.cpp:751-771): the estimate plus the reserve
 // (max(256 MiB, total / 10), src/gpu/gpu_memory.cpp:50-55) must fit in the free bytes. Host

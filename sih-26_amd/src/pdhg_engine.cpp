@@ -281,7 +281,7 @@ Solution run(const Model& model, const Options& options, Logger& logger, SolveCo
   report(model, res, y, reduced,
          ReportState{converged, stop_status, st.iteration, st.restarts, tolerance, stop_at_request,
                      final_params.stop_rule == 1},
-         &solution);
+         &solution, &prep);
   const double t_report = timer.elapsed_seconds();
   pdhg_dev_destroy(dev.h);
   dev.h = nullptr;

@@ -260,3 +260,25 @@ def test_interior_point_polish_handoff(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "all checks passed" in r.stdout
+
+
+@pytest.mark.parametrize("kind,m,n,per_col,seed", [("zipf", 2000, 5000, 10, 3), ("uniform", 1000, 3000, 5, 1),
+                                                   ("zipf", 20000, 50000, 10, 4)])
+def test_threaded_quality_equals_the_serial_pass_bit_for_bit(kind, m, n, per_col, seed):
+    """The report's threaded recompute_quality (row activities and scales through the CSR of A,
+    implied reduced costs by columns, maxima serial) against Solution::recompute_quality, which
+    the oracle tests pin to the reference's compiled model.cpp: the same bits, on points with
+    exact zeros, negative entries, values outside the bounds and row duals of both signs, on 1,
+    3 and 8 threads."""
+    mdl = sh.Model.generate(kind, m, n, per_col, seed)
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-2.0, 12.0, n)
+    x[rng.random(n) < 0.3] = 0.0
+    y = rng.standard_normal(m)
+    y[rng.random(m) < 0.2] = 0.0
+    d = rng.standard_normal(n)
+    act0, q0 = sh.quality(mdl, x, y, d, -123.25)
+    for threads in (1, 3, 8):
+        act, q = sh.quality(mdl, x, y, d, -123.25, threads=threads)
+        assert np.array_equal(act, act0), threads
+        assert np.array_equal(q, q0), (threads, q, q0)
