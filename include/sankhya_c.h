@@ -4,9 +4,9 @@
  * reference's name, arguments, return codes and contract (src/api/c_api.cpp); a caller written
  * against SANKHYA's C API links this library instead and runs the MI355X PDHG engine.
  *
- * What is left out: MPS reading (sankhya_model_read), quadratic objectives, warm starts from a
- * basis (sankhya_solve_from), basis statuses, certificates, ranging, IIS and the solution pool,
- * which belong to SANKHYA's other engines. The model handle is the same sankhya_model as in
+ * What is left out: CPLEX LP and QPS reading, quadratic objectives, warm starts from a basis
+ * (sankhya_solve_from), basis statuses, certificates, ranging, IIS and the solution pool, which
+ * belong to SANKHYA's other engines. The model handle is the same sankhya_model as in
  * sankhya_pdhg.h, so a model made from arrays (sankhya_model_new) can be solved here too.
  *
  * Every engine option of the registry is accepted by the sankhya_options_set_* calls, with the
@@ -61,6 +61,10 @@ double sankhya_infinity(void);
 
 /* sankhya.h:96-188: an empty model, then columns, rows and coefficients. */
 sankhya_model* sankhya_model_create(void);
+/* sankhya.h:99-106: read an MPS file (free or fixed) into the handle, replacing it. A `.lp`
+ * file, a quadratic section or gzip input fails with SANKHYA_ERROR_IO; on any failure the
+ * handle keeps what it held and sankhya_last_error() names the file and line. */
+sankhya_status sankhya_model_read(sankhya_model* model, const char* path);
 sankhya_status sankhya_model_set_maximize(sankhya_model* model, int maximize);
 sankhya_status sankhya_model_set_objective_offset(sankhya_model* model, double offset);
 sankhya_status sankhya_model_add_column(sankhya_model* model, double cost, double lower,

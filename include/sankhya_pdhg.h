@@ -50,6 +50,15 @@ void sankhya_model_free(sankhya_model* model);
 int sankhya_model_set_names(sankhya_model* model, const char* name, const char* source_path,
                             const char* const* col_names, const char* const* row_names);
 
+/* The names back: the model's name (kind 0), its source path (1), column `index`'s name (2) or
+ * row `index`'s name (3); NULL when there is none or the index is out of range. The pointer
+ * stays valid until the model is changed or freed. */
+const char* sankhya_model_name_of(const sankhya_model* model, int32_t kind, int32_t index);
+
+/* 1 for an integer column (MPS MARKER, BV, LI or UI), 0 for a continuous one, -1 when `col` is
+ * out of range. The PDHG engine solves the relaxation either way. */
+int32_t sankhya_model_col_is_integer(const sankhya_model* model, int32_t col);
+
 typedef struct {
   int32_t status; /* sankhya::SolveStatus */
   int64_t iterations, attempts, restarts, ticks, device_bytes;

@@ -113,6 +113,12 @@ def load_library() -> C.CDLL:
     lib.sankhya_batch_adjoint_masks.argtypes = [vp, _f64p, _f64p]
     lib.sankhya_model_set_names.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p),
                                             C.POINTER(C.c_char_p)]
+    lib.sankhya_model_name_of.argtypes = [vp, C.c_int32, C.c_int32]
+    lib.sankhya_model_name_of.restype = C.c_char_p
+    lib.sankhya_model_col_is_integer.argtypes = [vp, C.c_int32]
+    lib.sankhya_model_create.restype = vp
+    lib.sankhya_model_read.argtypes = [vp, C.c_char_p]
+    lib.sankhya_last_error.restype = C.c_char_p
     lib.sankhya_write_solution.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(Result), vp, vp,
                                            vp, vp, C.c_char_p, C.c_int32]
     lib.sankhya_pdhg_scaling.argtypes = [vp, C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p,
@@ -196,6 +202,29 @@ class Model:
         cu = np.full(n, np.inf) if col_upper is None else col_upper
         return cls.from_csc(m, n, colptr, rowidx, val, cost, cl, cu, row_lower, row_upper,
                             objective_offset, maximize)
+
+    @classmethod
+    def read_mps(cls, path: str) -> "Model":
+        """An MPS file (free or fixed) read by the library's reader (sankhya_model_read)."""
+        lib = load_library()
+        h = lib.sankhya_model_create()
+        model = cls(h)
+        if lib.sankhya_model_read(model._h, os.fspath(path).encode()) != 0:
+            raise ValueError(lib.sankhya_last_error().decode())
+        return model
+
+    def names(self) -> Dict[str, object]:
+        """The model's name, source path and column and row names (empty strings when unset)."""
+        m, n, _ = self.dims
+        get = lambda kind, i=0: (self._lib.sankhya_model_name_of(self._h, kind, i) or b"").decode()
+        return dict(name=get(0), source_path=get(1), col_names=[get(2, j) for j in range(n)],
+                    row_names=[get(3, i) for i in range(m)])
+
+    def integer_columns(self) -> np.ndarray:
+        """Indices of the columns marked integer (the PDHG engine solves the relaxation)."""
+        n = self.dims[1]
+        return np.array([j for j in range(n) if self._lib.sankhya_model_col_is_integer(self._h, j) == 1],
+                        dtype=np.int64)
 
     @classmethod
     def generate(cls, kind: str, m: int, n: int, per_col: int, seed: int, param: float = 1.5) -> "Model":

@@ -317,6 +317,27 @@ int sankhya_model_set_names(sankhya_model* h, const char* name, const char* sour
   return 0;
 }
 
+const char* sankhya_model_name_of(const sankhya_model* h, int32_t kind, int32_t index) {
+  if (!h) return nullptr;
+  const Model& md = h->model;
+  auto pick = [index](const std::vector<std::string>& names) -> const char* {
+    if (index < 0 || static_cast<std::size_t>(index) >= names.size()) return nullptr;
+    return names[static_cast<std::size_t>(index)].c_str();
+  };
+  switch (kind) {
+    case 0: return md.name.c_str();
+    case 1: return md.source_path.c_str();
+    case 2: return pick(md.col_names);
+    case 3: return pick(md.row_names);
+    default: return nullptr;
+  }
+}
+
+int32_t sankhya_model_col_is_integer(const sankhya_model* h, int32_t col) {
+  if (!h || col < 0 || static_cast<std::size_t>(col) >= h->model.col_type.size()) return -1;
+  return h->model.col_type[static_cast<std::size_t>(col)] == VarType::kInteger ? 1 : 0;
+}
+
 int sankhya_write_solution(const sankhya_model* h, const char* options, const char* path,
                            const sankhya_result* result, const double* col_value,
                            const double* row_dual, const double* col_dual,

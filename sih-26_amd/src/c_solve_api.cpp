@@ -10,10 +10,12 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/sankhya_c.h"
 #include "c_model.hpp"
+#include "sankhya/io.hpp"
 #include "sankhya/options.hpp"
 #include "sankhya/pdhg.hpp"
 
@@ -132,6 +134,20 @@ sankhya_model* sankhya_model_create(void) {
     g_last_error = "out of memory";
     return nullptr;
   }
+}
+
+// src/api/c_api.cpp:185-200: on failure the handle keeps what it held.
+sankhya_status sankhya_model_read(sankhya_model* h, const char* path) {
+  if (!h || !path) return fail(SANKHYA_ERROR_ARGUMENT, "model or path is null");
+  return guarded([&] {
+    Model fresh;
+    const io::ReadResult result = io::read_model(path, &fresh);
+    if (!result.ok) return fail(SANKHYA_ERROR_IO, result.error);
+    h->model = std::move(fresh);
+    h->entries.clear();
+    h->pending = false;
+    return ok();
+  });
 }
 
 sankhya_status sankhya_model_set_maximize(sankhya_model* h, int maximize) {

@@ -222,9 +222,11 @@ def test_verifier_restatement_rejects_planted_errors():
 
 REFERENCE = "/root/reference"
 # The engine's drop-in sources: the driver behind sankhya::pdhg::solve_pdhg /
-# sankhya::gpu::solve_pdhg_gpu, its host steps, the batched driver and the threaded
-# preconditioner. SANKHYA builds them against its own headers, not against the mirror here.
-SEAM_SOURCES = ["pdhg_engine.cpp", "engine_common.cpp", "pdhg_batch_engine.cpp", "scaling.cpp"]
+# sankhya::gpu::solve_pdhg_gpu, its host steps, the batched driver, the threaded
+# preconditioner and the MPS reader (sankhya::io::read_mps). SANKHYA builds them against its
+# own headers, not against the mirror here.
+SEAM_SOURCES = ["pdhg_engine.cpp", "engine_common.cpp", "pdhg_batch_engine.cpp", "scaling.cpp",
+                "mps_reader.cpp"]
 
 
 @pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE, "include", "sankhya")),
