@@ -4,13 +4,13 @@
 // CSR (rows of Ahat, and separately the rows of Ahat^T, i.e. the columns of Ahat), and a
 // host-built plan cuts it into work items, one per 256-thread workgroup:
 //
-//   stream item  - up to kTile nonzeros covering whole rows [r0, r1). The workgroup streams
+//   stream item  - fewer than kTile nonzeros covering whole rows [r0, r1). The workgroup streams
 //                  values and indices with unit stride, multiplies each by the gathered
 //                  right-hand side and stages the products in LDS; then each row is summed by
 //                  a power-of-two group of lanes (1..64) sized to the rows in the item, and
 //                  reduced with cross-lane shuffles. Short rows therefore never leave a lane
 //                  idle on a coalesced load.
-//   long item    - one segment of kSeg nonzeros of a row longer than kTile (the power-law
+//   long item    - one segment (< kSeg nonzeros) of a row of kTile or more (the power-law
 //                  rows of a Zipf-distributed pattern). The workgroup writes the segment's
 //                  sum and nothing else. A single "tail" workgroup launched afterwards adds each
 //                  row's segment sums in index order (staged through LDS) and runs that row's
@@ -37,7 +37,7 @@ constexpr int kTile = 2048;     // nonzeros per stream item
 constexpr int kSeg = 4096;      // nonzeros per long-row segment
 constexpr int kTail = 1024;     // threads of the single-workgroup tail kernels
 constexpr int kChunk = 2048;    // long-row segment sums staged in LDS per tail round
-constexpr int kSegBatch = 9;    // long-row segment steps whose gathers are issued together
+constexpr int kSegBatch = 8;    // long-row segment steps whose gathers are issued together
 constexpr int kMaxRows = 512;   // rows per stream item
 
 __device__ __forceinline__ bool finite_bound(double v) {
@@ -417,8 +417,9 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
     const int grp = tid / L, lane = tid & (L - 1), ngrp = kThreads / L;
     // Two nonzeros per lane per step (one 16-byte value load, one 8-byte index load), from a
     // pair-aligned base; kTile / (2 kThreads) steps, all loads issued before the first gather.
+    // The plan keeps an item below kTile nonzeros, so its span from the aligned base fits.
     const int kb = k0 & ~1;
-    constexpr int U = kTile / (2 * kThreads) + 1;
+    constexpr int U = kTile / (2 * kThreads);
     int2 cidx[U];
     double2 aval[U];
 #pragma unroll
@@ -515,7 +516,8 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
     return;
   }
   // Long-row segment {global segment id, -(long row id)-1, k0, k1}: kSeg / kThreads
-  // independent loads per lane, then the gathers.
+  // independent loads per lane (a segment is below kSeg nonzeros, so its span from the aligned
+  // base fits kSeg), then the gathers.
   const int seg_id = it.x, k0 = it.z, k1 = it.w;
   Acc<NR, 0> seg;
 #pragma unroll
@@ -529,7 +531,7 @@ __device__ void run_item_geo(const PlanView<V>& P, const int4 it, G& gather, E& 
   const int4 info = P.lr_info[lr];
   {
     const int kb = k0 & ~1;
-    constexpr int U = kSeg / (2 * kThreads) + 1;
+    constexpr int U = kSeg / (2 * kThreads);
     int2 cidx[U];
     double2 aval[U];
 #pragma unroll

@@ -21,9 +21,12 @@ struct HostPlan {
   int32_t nseg = 0;
 };
 
-// Work items over a CSR matrix: whole rows packed up to kTile nonzeros and kMaxRows rows per
-// item; rows longer than kTile split into segments of at most kSeg nonzeros that never cross a
-// column band (ncols split into kXcds equal bands).
+// Work items over a CSR matrix: whole rows packed below kTile nonzeros and up to kMaxRows rows
+// per item; rows of kTile nonzeros or more split into segments of fewer than kSeg nonzeros that
+// never cross a column band (ncols split into kXcds equal bands). Staying one short of kTile and
+// kSeg keeps an item's span from its pair-aligned first nonzero within kTile (kSeg), so the
+// kernels cover it in exactly kTile / (2 threads) two-nonzero steps: an odd start would
+// otherwise need one step more, which costs every lane its registers whether used or not.
 //
 // XCD-aware dispatch: workgroup b runs on XCD b % kXcds, and each XCD has its own L2. The
 // segments of band k are placed at item positions congruent to k, so every XCD's L2 only has to
@@ -79,7 +82,7 @@ inline HostPlan build_plan(int32_t rows, int32_t ncols, const int32_t* rowptr, c
   int32_t r = 0;
   while (r < rows) {
     const int32_t len = rowptr[r + 1] - rowptr[r];
-    if (len > g.tile) {
+    if (len >= g.tile) {
       const int32_t lr = static_cast<int32_t>(hp.lr_nseg.size());
       int32_t nseg = 0;
       int32_t k = rowptr[r];
@@ -88,8 +91,8 @@ inline HostPlan build_plan(int32_t rows, int32_t ncols, const int32_t* rowptr, c
         const int64_t band_end_col = (b + 1) * width;
         int32_t kb = k;  // first nonzero of the next band
         while (kb < rowptr[r + 1] && col[kb] < band_end_col) ++kb;
-        for (int32_t k0 = k; k0 < kb; k0 += g.seg) {
-          const int32_t k1 = std::min(k0 + g.seg, kb);
+        for (int32_t k0 = k; k0 < kb; k0 += g.seg - 1) {
+          const int32_t k1 = std::min(k0 + g.seg - 1, kb);
           const int4 item = make_int4(hp.nseg + nseg, -lr - 1, k0, k1);
           by_band[b].push_back(item);
           logical->push_back(item);
@@ -109,7 +112,7 @@ inline HostPlan build_plan(int32_t rows, int32_t ncols, const int32_t* rowptr, c
     int32_t longest = 0;
     while (r < rows && r - r0 < g.max_rows) {
       const int32_t l = rowptr[r + 1] - rowptr[r];
-      if (l > g.tile || acc + l > g.tile) break;
+      if (acc + l >= g.tile) break;
       const int32_t lg = std::max(longest, l);
       const int L = lanes_per_row(r - r0 + 1, g.threads);
       if (r > r0 && (lg + L - 1) / L > kSerialSum) break;
@@ -167,8 +170,8 @@ inline HostPlan build_plan(int32_t rows, int32_t ncols, const int32_t* rowptr, c
 }
 
 
-// Every row covered exactly once (in the logical order), stream items within the LDS tile
-// and the row cap, long segments within kSeg and numbered consecutively per row, and the
+// Every row covered exactly once (in the logical order), stream items below the LDS tile and
+// within the row cap, long segments below kSeg and numbered consecutively per row, and the
 // dispatch order a permutation of the logical one: checked on the host before any kernel sees
 // the plan.
 inline bool plan_ok(const HostPlan& hp, const std::vector<int4>& logical, int32_t rows,
@@ -178,7 +181,7 @@ inline bool plan_ok(const HostPlan& hp, const std::vector<int4>& logical, int32_
     if (it.y >= 0) {
       if (next_k >= 0) return false;
       if (it.x != next_row || it.y <= it.x || it.y - it.x > g.max_rows) return false;
-      if (it.z != rowptr[it.x] || it.w != rowptr[it.y] || it.w - it.z > g.tile) return false;
+      if (it.z != rowptr[it.x] || it.w != rowptr[it.y] || it.w - it.z >= g.tile) return false;
       next_row = it.y;
     } else {
       const int32_t lr = -it.y - 1;
@@ -186,7 +189,7 @@ inline bool plan_ok(const HostPlan& hp, const std::vector<int4>& logical, int32_
       const int32_t row = hp.lr_row[lr];
       if (row != next_row) return false;
       if (it.x < hp.lr_base[lr] || it.x >= hp.lr_base[lr] + hp.lr_nseg[lr]) return false;
-      if (it.z != (next_k < 0 ? rowptr[row] : next_k) || it.w <= it.z || it.w - it.z > g.seg) return false;
+      if (it.z != (next_k < 0 ? rowptr[row] : next_k) || it.w <= it.z || it.w - it.z >= g.seg) return false;
       next_k = it.w;
       if (it.w == rowptr[row + 1]) {
         next_row = row + 1;
