@@ -44,7 +44,15 @@
 
 namespace sankhya_hip {
 
-constexpr int kDefaultChunk = 48;
+// Attempt slots per graph chunk: the evaluation interval and two spare slots. A tick is due
+// after kEvalInterval accepted attempts, and nearly every attempt is accepted (20,512 attempts
+// for 20,400 iterations on the 200k x 500k Zipf LP), so the tick usually falls in the chunk's
+// last three slots. Every slot after it still launches its three kernels, which exit at once
+// (about 7.5 us a slot). A cycle that needs more attempts than the chunk holds spills into a
+// second chunk. Measured per iteration: 42 slots 56.7 us, 40 57.6, 48 57.8, 64 60.8.
+constexpr int kDefaultChunk = static_cast<int>(kEvalInterval) + 2;
+
+constexpr size_t kBlockBytes = size_t{64} << 20;  // smallest device block a handle allocates
 
 // Workgroup sizes of the two iteration kernels (tiles as in SingleGeo).
 constexpr int kPrimalThreads = 256;
@@ -459,13 +467,32 @@ __global__ __launch_bounds__(kThreads) void k_sc_items(const int4* items, const 
   const int4 it = items[blockIdx.x];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (it.y >= 0) {
-    for (int32_t row = it.x + wave; row < it.y; row += kThreads / 64) {
+    // A stream item's rows, L lanes each as in the products (L the largest power of two with
+    // rows * L <= kThreads), so that all of them are in flight together: one wave per row
+    // walked the Zipf LP's items of ~290 three-entry rows one memory round trip per row. The
+    // maxima are exact in any order and each entry's (a * r) * c is computed as before, so the
+    // results are unchanged bit for bit.
+    const int rows = it.y - it.x;
+    int L = 64;
+    while (L > 1 && rows * L > kThreads) L >>= 1;
+    const int grp = threadIdx.x / L, gl = threadIdx.x & (L - 1), ngrp = kThreads / L;
+    for (int rr = grp; rr < rows; rr += ngrp) {  // the L lanes of a group stay together
+      const int32_t row = it.x + rr;
       const int32_t k0 = ptr[row], k1 = ptr[row + 1];
       if (mode == 0) {
-        const double r = wave_absmax(val, k0, k1, lane);
-        if (lane == 0) stat[row] = r;
+        double r = 0.0;
+#pragma unroll 4
+        for (int32_t k = k0 + gl; k < k1; k += L) r = smax(r, fabs(val[k]));
+        for (int off = L >> 1; off > 0; off >>= 1) r = smax(r, __shfl_xor(r, off));
+        if (gl == 0) stat[row] = r;
       } else {
-        wave_apply(idx, val, k0, k1, lane, line_pass[row], idx_pass, line_is_row);
+        const double lf = line_pass[row];
+#pragma unroll 4
+        for (int32_t k = k0 + gl; k < k1; k += L) {
+          const double other = idx_pass[idx[k]];
+          const double r = line_is_row ? lf : other, c = line_is_row ? other : lf;
+          val[k] = val[k] * r * c;
+        }
       }
     }
     return;
@@ -667,7 +694,12 @@ struct pdhg_dev {
   bool cached = false;
   bool uploaded = false, started = false;
   int64_t bytes = 0;
+  // Device memory comes in blocks of at least kBlockBytes, carved into 256-byte aligned pieces
+  // and freed together with the handle: a few hipMalloc and hipFree calls per solve instead of
+  // one per array (about 45), which took the 200k x 500k LP's release from 6-7 ms.
   std::vector<void*> allocs;
+  char* block_next = nullptr;
+  size_t block_left = 0;
 };
 
 namespace {
@@ -703,11 +735,19 @@ template <typename T>
 int dalloc(pdhg_dev* h, T** p, size_t count) {
   *p = nullptr;
   const size_t bytes = (count + kSpare) * sizeof(T);
-  if (hipMalloc(reinterpret_cast<void**>(p), bytes) != hipSuccess) {
-    *p = nullptr;
-    return fail(h, PDHG_E_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  const size_t take = (bytes + 255) & ~size_t{255};
+  if (take > h->block_left) {
+    const size_t block = std::max(take, kBlockBytes);
+    void* b = nullptr;
+    if (hipMalloc(&b, block) != hipSuccess)
+      return fail(h, PDHG_E_ALLOC, "hipMalloc of " + std::to_string(block) + " bytes failed");
+    h->allocs.push_back(b);
+    h->block_next = static_cast<char*>(b);
+    h->block_left = block;
   }
-  h->allocs.push_back(*p);
+  *p = reinterpret_cast<T*>(h->block_next);
+  h->block_next += take;
+  h->block_left -= take;
   h->bytes += static_cast<int64_t>(bytes);
   HIP_TRY(h, hipMemsetAsync(*p + count, 0, kSpare * sizeof(T), h->stream));
   return PDHG_OK;
@@ -1178,9 +1218,16 @@ int pdhg_dev_upload(pdhg_dev* h, int32_t m, int32_t n, int64_t nnz, const int32_
     if (a_rowptr[i + 1] < a_rowptr[i]) return fail(h, PDHG_E_ARGUMENT, "row pointer not monotone");
   for (int32_t j = 0; j < n; ++j)
     if (at_rowptr[j + 1] < at_rowptr[j]) return fail(h, PDHG_E_ARGUMENT, "transpose pointer not monotone");
-  for (int64_t k = 0; k < nnz; ++k)
-    if (a_colidx[k] < 0 || a_colidx[k] >= n || at_colidx[k] < 0 || at_colidx[k] >= m)
-      return fail(h, PDHG_E_ARGUMENT, "index out of range");
+  // One unsigned comparison per index (a negative index wraps above any bound), folded without
+  // a branch so that the loop vectorises: 10M indices in well under a millisecond.
+  {
+    const uint32_t un = static_cast<uint32_t>(n), um = static_cast<uint32_t>(m);
+    uint32_t bad = 0;
+    for (int64_t k = 0; k < nnz; ++k)
+      bad |= static_cast<uint32_t>(static_cast<uint32_t>(a_colidx[k]) >= un) |
+             static_cast<uint32_t>(static_cast<uint32_t>(at_colidx[k]) >= um);
+    if (bad) return fail(h, PDHG_E_ARGUMENT, "index out of range");
+  }
   if (injected(h)) return PDHG_E_INJECTED;
   HIP_TRY(h, hipSetDevice(h->device));
   h->m = m;

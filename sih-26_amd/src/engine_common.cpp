@@ -36,8 +36,8 @@ Residuals from_array(const double* r) {
 // CSR of a CSC pattern by a counting-sort transpose: column indices ascend within each row
 // (reference CsrView::build, src/la/sparse.cpp:271-322).
 void csc_to_csr(Index m, Index n, const std::vector<Index>& cp, const std::vector<Index>& ri,
-                const std::vector<double>& v, std::vector<Index>* rp, std::vector<Index>* ci,
-                std::vector<double>* cv, int threads) {
+                const std::vector<double>& v, std::vector<Index>* rp, HostArray<Index>* ci,
+                HostArray<double>* cv, int threads) {
   const std::size_t nnz = v.size(), um = static_cast<std::size_t>(m), un = static_cast<std::size_t>(n);
   if (nnz < 200000) threads = 1;
   threads = std::max(1, threads);
@@ -140,7 +140,7 @@ std::vector<double> power_start(const Model& model, std::int64_t random_seed) {
 }
 
 void recompute_quality_threaded(const Model& model, const std::vector<Index>& rp,
-                                const std::vector<Index>& ci, const std::vector<double>& cv,
+                                const HostArray<Index>& ci, const HostArray<double>& cv,
                                 int threads, Solution* s) {
   const Index n = model.num_cols(), m = model.num_rows();
   if (static_cast<Index>(s->col_value.size()) != n) return;

@@ -6,7 +6,10 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "sankhya/model.hpp"
@@ -34,20 +37,45 @@ struct Residuals {
 };
 Residuals from_array(const double* r);
 
+// Storage whose resize leaves new elements of a trivial type uninitialised, for arrays written
+// whole right afterwards: zero-filling the fresh 60 MB of a 5M-nonzero CSR on one thread (the
+// first touch of every page) took longer than the parallel transpose that then fills it.
+template <class T>
+struct UninitAllocator : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = UninitAllocator<U>;
+  };
+  UninitAllocator() noexcept = default;
+  template <class U>
+  UninitAllocator(const UninitAllocator<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept(std::is_nothrow_default_constructible_v<U>) {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+  }
+};
+template <class T>
+using HostArray = std::vector<T, UninitAllocator<T>>;
+
 // CSR of a CSC matrix, columns ascending within each row (the counting-sort transpose of
 // src/la/sparse.cpp:271-302); threads > 1 gives the same arrays (matrices under 200,000
 // nonzeros use one thread).
 void csc_to_csr(Index m, Index n, const std::vector<Index>& cp, const std::vector<Index>& ri,
-                const std::vector<double>& v, std::vector<Index>* rp, std::vector<Index>* ci,
-                std::vector<double>* cv, int threads = 1);
+                const std::vector<double>& v, std::vector<Index>* rp, HostArray<Index>* ci,
+                HostArray<double>* cv, int threads = 1);
 
 // One LP as the device receives it.
 struct Prepared {
   std::vector<double> cost;  // minimise-space cost
   double cost_norm = 0.0, bound_norm = 0.0;
   Scaling scaling;
-  std::vector<Index> a_rp, a_ci;  // CSR of Ahat
-  std::vector<double> a_v;
+  std::vector<Index> a_rp;  // CSR of Ahat
+  HostArray<Index> a_ci;
+  HostArray<double> a_v;
   double scaling_seconds = 0.0, csr_seconds = 0.0;  // host time of the two steps
 };
 // threads: host threads for the preconditioner (the batch prepares LPs in parallel with 1).
@@ -80,7 +108,7 @@ void report(const Model& model, const double* res9, const std::vector<double>& y
 // columns; the passes that take maxima over rows or columns stay serial, in the reference's
 // order. The quality fields equal the serial function's bit for bit.
 void recompute_quality_threaded(const Model& model, const std::vector<Index>& rp,
-                                const std::vector<Index>& ci, const std::vector<double>& cv,
+                                const HostArray<Index>& ci, const HostArray<double>& cv,
                                 int threads, Solution* s);
 
 // The dispatcher's VRAM decision (src/core/solve.cpp:751-771): the estimate plus the reserve
