@@ -327,6 +327,7 @@ __device__ __forceinline__ void batch_line(const BatchLines& L, int64_t line, in
 
 // mode 0: largest |a| of each line into stat; 1: Pock-Chambolle sum of |a| in storage order.
 __global__ __launch_bounds__(kThreads) void k_bsc_stat(BatchLines L, int mode, double* stat) {
+  __shared__ double stage[kThreads * kScU];
   const int lane = threadIdx.x & 63;
   const int64_t line = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
   if (line >= L.lines) return;  // the whole wave leaves together
@@ -335,7 +336,8 @@ __global__ __launch_bounds__(kThreads) void k_bsc_stat(BatchLines L, int mode, d
   batch_line(L, line, k0, k1, base, other0);
   const double* v = L.val + base;
   const double r = mode == 0 ? wave_absmax(v, k0, k1, lane)
-                             : wave_abssum_ordered(v, k0, k1, lane);
+                             : wave_abssum_ordered(v, k0, k1, lane,
+                                                   stage + (threadIdx.x >> 6) * 64 * kScU);
   if (lane == 0) stat[line] = r;
 }
 

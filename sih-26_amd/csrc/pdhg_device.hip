@@ -531,10 +531,12 @@ __global__ __launch_bounds__(kThreads) void k_sc_longmax(const int4* lr_info, in
 // entries too: its sum is one sequential chain).
 __global__ __launch_bounds__(kThreads) void k_sc_linesum(const int32_t* ptr, const double* val,
                                                          int32_t lines, double* stat) {
+  __shared__ double stage[kThreads * kScU];
   const int lane = threadIdx.x & 63;
   const int64_t line = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
   if (line >= lines) return;  // the whole wave leaves together
-  const double r = wave_abssum_ordered(val, ptr[line], ptr[line + 1], lane);
+  const double r = wave_abssum_ordered(val, ptr[line], ptr[line + 1], lane,
+                                       stage + (threadIdx.x >> 6) * 64 * kScU);
   if (lane == 0) stat[line] = r;
 }
 

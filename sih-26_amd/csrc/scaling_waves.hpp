@@ -3,7 +3,7 @@
 // matrix line for the Ruiz maxima, the scaling of the entries and the Pock-Chambolle sums in
 // storage order, and the flat per-coordinate kernels (factors, fills, scaled vectors). The
 // results equal the reference's host passes bit for bit: maxima do not depend on order, the
-// sums are added in storage order by one lane, and factors and scaled values use the same
+// sums are added in storage order as one chain, and factors and scaled values use the same
 // correctly rounded IEEE double operations in the same order.
 #pragma once
 
@@ -62,23 +62,20 @@ __device__ __forceinline__ void wave_apply(const int32_t* idx, double* val, int3
   }
 }
 
-// One lane's double as a wave-uniform value (two v_readlane_b32): lane `l` must be uniform.
-__device__ __forceinline__ double readlane_f64(double x, int l) {
-  const long long b = __double_as_longlong(x);
-  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b), l));
-  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), l));
-  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
-}
-
 // Sum of |a| over [k0, k1) in storage order, one wave, as one sequential chain of adds, so the
 // result is the sequential sum bit for bit. Each step's 64 kScU entries arrive with coalesced
-// loads; the loads of the next step are issued before the current step's entries are added,
-// so only the chain of adds (entry e is lane e % 64 of register e / 64, read with v_readlane)
-// is on the critical path, not a memory round trip per step. The chain walks blocks of 8 lanes
-// up to the last valid entry; entries past k1 in the last block are +0.0 and leave the sum (a
-// sum of magnitudes, never -0.0) unchanged. Every lane returns the sum.
+// loads and are staged in `stage` (this wave's 64 kScU doubles of LDS); the loads of the next
+// step are issued before the current step's entries are added. Every lane then reads the staged
+// entries in storage order (one broadcast LDS read per entry) and runs the same chain, so only
+// the dependent adds are on the critical path: on the densest row of the 200k x 500k Zipf LP
+// (about 495,000 entries) reading the entries back with v_readlane instead took 6.9 ms.
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __device__ __forceinline__ double wave_abssum_ordered(const double* val, int32_t k0, int32_t k1,
-                                                      int lane) {
+                                                      int lane, double* stage) {
   double r = 0.0;
   if (k1 <= k0) return r;
   // Loads are never conditional (a lane past the end reads entry k1 - 1 of this line and
@@ -99,14 +96,32 @@ __device__ __forceinline__ double wave_abssum_ordered(const double* val, int32_t
       const int32_t k = next + u * 64 + lane;
       nv[u] = fabs(val[k < last ? k : last]) * (k < k1 ? 1.0 : 0.0);
     }
+#pragma unroll
+    for (int u = 0; u < kScU; ++u) stage[u * 64 + lane] = v[u];  // entry base + i at stage[i]
+    wave_sync_lds();
     const int32_t count = k1 - base < 64 * kScU ? k1 - base : 64 * kScU;
+    // Groups of 16, the next group's reads issued before the current group's adds.
+    const int32_t full = count & ~15;
+    int32_t i = 0;
+    if (full > 0) {
+      double t[16];
 #pragma unroll
-    for (int u = 0; u < kScU; ++u) {
-      for (int lb = 0; lb < 64 && u * 64 + lb < count; lb += 8) {
+      for (int q = 0; q < 16; ++q) t[q] = stage[q];
+#pragma unroll 1
+      for (i = 16; i < full; i += 16) {
+        double nt[16];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) r += readlane_f64(v[u], lb + q);
+        for (int q = 0; q < 16; ++q) nt[q] = stage[i + q];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) r += t[q];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) t[q] = nt[q];
       }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) r += t[q];
     }
+    for (; i < count; ++i) r += stage[i];
+    wave_sync_lds();  // every lane has read the step before the next one is staged
 #pragma unroll
     for (int u = 0; u < kScU; ++u) v[u] = nv[u];
   }
