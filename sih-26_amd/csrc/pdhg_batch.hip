@@ -81,10 +81,40 @@ __device__ inline PlanView<V> lp_view(const Mat<V>& M, const PlanArrays& P, cons
   return v;
 }
 
+// One work item of a cooperative grid with what its workgroup needs of its LP: the item in its
+// LP's numbering, its index among the geometry-0 items (its partial sums), the LP, and the LP's
+// offsets into the concatenated arrays on the item's matrix side. Filled on the host when the
+// grids are compacted, so that a workgroup's loads wait for this one uniform record instead of
+// the item's index, then the item and its LP, then the LP's descriptor.
+struct alignas(16) CoopItem {
+  int4 item;
+  int32_t b, lp, rp0, lr0;
+  int64_t nnz0;
+  int32_t seg0, row0, col0, m;
+  int32_t n, pad;
+};
+static_assert(sizeof(CoopItem) == 64, "one 64-byte record per cooperative work item");
+
+template <typename V>
+__device__ inline PlanView<V> item_view(const Mat<V>& M, const PlanArrays& P, const CoopItem& r) {
+  PlanView<V> v;
+  v.rowptr = M.rowptr + r.rp0;
+  v.col = M.col + r.nnz0;
+  v.val = M.val + r.nnz0;
+  v.items = P.items;  // an item kernel is handed its item and walks no other
+  v.lr_info = P.lr_info + r.lr0;
+  v.lr_groups = P.groups;
+  v.seg_part = P.seg_part + 2 * static_cast<int64_t>(r.seg0);
+  v.nitems = 0;  // the item kernels walk only their own item
+  v.nlr = 0;
+  v.ngroups = 0;
+  v.has_group0 = 0;
+  return v;
+}
+
 template <typename X>
-__device__ inline Iter<X> lp_iter(const Iter<X>& g, const LpDesc& d) {
+__device__ inline Iter<X> lp_iter(const Iter<X>& g, int64_t r, int64_t c, int32_t m, int32_t n) {
   Iter<X> it = g;
-  const int64_t c = d.col0, r = d.row0;
   it.xbuf = g.xbuf + 2 * c;
   it.ybuf = g.ybuf + 2 * r;
   it.xsum = g.xsum + c;
@@ -107,9 +137,13 @@ __device__ inline Iter<X> lp_iter(const Iter<X>& g, const LpDesc& d) {
   it.ru = g.ru + r;
   it.axr = g.axr + r;
   it.atyr = g.atyr + c;
-  it.n = d.n;
-  it.m = d.m;
+  it.n = n;
+  it.m = m;
   return it;
+}
+template <typename X>
+__device__ inline Iter<X> lp_iter(const Iter<X>& g, const LpDesc& d) {
+  return lp_iter(g, static_cast<int64_t>(d.row0), static_cast<int64_t>(d.col0), d.m, d.n);
 }
 
 // Everything a batch kernel reads, by value.
@@ -385,8 +419,8 @@ struct CoopArgs {
   const int32_t* col_lp;     // LP of each concatenated column
   const int32_t* row_lp;     // LP of each concatenated row
   const int32_t* coop;       // per LP: 1 when it runs in the cooperative schedule
-  const int32_t* act_a;      // the geometry-0 items of Ahat of the LPs still running (grid)
-  const int32_t* act_t;      // ... of Ahat^T
+  const CoopItem* act_a;     // the geometry-0 items of Ahat of the LPs still running (grid)
+  const CoopItem* act_t;     // ... of Ahat^T
   const int32_t* act_lp;     // the LPs still running (grid of the per-LP kernels)
   double *part_p, *part_d, *part_er, *part_ec, *lr_p;
 };
@@ -398,14 +432,13 @@ template <typename V, typename X>
 __global__ __launch_bounds__(kCT) void k_coop_primal(CoopArgs<V, X> C, int force) {
   __shared__ double lds[CG::kTile];
   __shared__ double red[4 * 2];
-  const int b = C.act_t[blockIdx.x];
-  const int4 item = C.a.pt.items[b];
-  const int lp = C.item_lp_t[b];
+  const CoopItem r = C.act_t[blockIdx.x];
+  const int b = r.b, lp = r.lp;
   const Ctl* ctl = C.a.ctl + lp;
   if (!C.coop[lp] || (!force && ctl->halt)) return;
-  const LpDesc& d = C.a.desc[lp];
-  const PlanView<V> T = lp_view(C.a.T, C.a.pt, d.t[0], d.nnz0, d.trp0);
-  const Iter<X> it = lp_iter(C.a.it, d);
+  const PlanView<V> T = item_view(C.a.T, C.a.pt, r);
+  const Iter<X> it = lp_iter(C.a.it, r.row0, r.col0, r.m, r.n);
+  const int4 item = r.item;
   PrimalGather<X> g{it.ybuf, ctl->cur};
   PrimalEpi<X> e = primal_epi(it, ctl);
   Acc<1, 0> acc;
@@ -438,14 +471,13 @@ template <typename V, typename X>
 __global__ __launch_bounds__(kCT) void k_coop_dual(CoopArgs<V, X> C, int force) {
   __shared__ double lds[CG::kTile];
   __shared__ double red[4 * 2];
-  const int b = C.act_a[blockIdx.x];
-  const int4 item = C.a.pa.items[b];
-  const int lp = C.item_lp_a[b];
+  const CoopItem r = C.act_a[blockIdx.x];
+  const int b = r.b, lp = r.lp;
   const Ctl* ctl = C.a.ctl + lp;
   if (!C.coop[lp] || (!force && ctl->halt)) return;
-  const LpDesc& d = C.a.desc[lp];
-  const PlanView<V> A = lp_view(C.a.A, C.a.pa, d.a[0], d.nnz0, d.arp0);
-  const Iter<X> it = lp_iter(C.a.it, d);
+  const PlanView<V> A = item_view(C.a.A, C.a.pa, r);
+  const Iter<X> it = lp_iter(C.a.it, r.row0, r.col0, r.m, r.n);
+  const int4 item = r.item;
   DualGather<X> g{it.xbuf, ctl->cur};
   DualEpi<X> e = dual_epi(it, ctl);
   Acc<2, 0> acc;
@@ -501,14 +533,13 @@ template <typename X>
 __global__ __launch_bounds__(kCT) void k_coop_eval_rows(CoopArgs<double, X> C) {
   __shared__ double lds[CG::kTile];
   __shared__ double red[4 * 9];
-  const int b = C.act_a[blockIdx.x];
-  const int4 item = C.a.pa.items[b];
-  const int lp = C.item_lp_a[b];
+  const CoopItem r = C.act_a[blockIdx.x];
+  const int b = r.b, lp = r.lp;
   const Ctl* ctl = C.a.ctl + lp;
   if (!C.coop[lp] || !ctl->tick) return;
-  const LpDesc& d = C.a.desc[lp];
-  const PlanView<double> A64 = lp_view(C.a.A64, C.a.pa, d.a[0], d.nnz0, d.arp0);
-  const Iter<X> it = lp_iter(C.a.it, d);
+  const PlanView<double> A64 = item_view(C.a.A64, C.a.pa, r);
+  const Iter<X> it = lp_iter(C.a.it, r.row0, r.col0, r.m, r.n);
+  const int4 item = r.item;
   const int cur = ctl->cur, tp = ctl->pending;
   const double count = tick_count(ctl);
   TickRowsGather<X> g{it, cur, tp, count};
@@ -527,14 +558,13 @@ template <typename X>
 __global__ __launch_bounds__(kCT) void k_coop_eval_cols(CoopArgs<double, X> C) {
   __shared__ double lds[CG::kTile];
   __shared__ double red[4 * 9];
-  const int b = C.act_t[blockIdx.x];
-  const int4 item = C.a.pt.items[b];
-  const int lp = C.item_lp_t[b];
+  const CoopItem r = C.act_t[blockIdx.x];
+  const int b = r.b, lp = r.lp;
   const Ctl* ctl = C.a.ctl + lp;
   if (!C.coop[lp] || !ctl->tick) return;
-  const LpDesc& d = C.a.desc[lp];
-  const PlanView<double> T64 = lp_view(C.a.T64, C.a.pt, d.t[0], d.nnz0, d.trp0);
-  const Iter<X> it = lp_iter(C.a.it, d);
+  const PlanView<double> T64 = item_view(C.a.T64, C.a.pt, r);
+  const Iter<X> it = lp_iter(C.a.it, r.row0, r.col0, r.m, r.n);
+  const int4 item = r.item;
   const int cur = ctl->cur, tp = ctl->pending;
   const double count = tick_count(ctl);
   TickColsGather<X> g{it, cur, tp, count};
@@ -965,8 +995,11 @@ struct pdhg_batch {
   int32_t* coop_dev = nullptr;
   std::vector<int32_t> coop;  // per LP: 1 cooperative, 0 persistent (geometry in geo)
   int32_t items_a = 0, items_t = 0;
-  // Compacted grids: only the LPs still running (rebuilt as LPs stop).
-  int32_t *act_a = nullptr, *act_t = nullptr, *act_lp = nullptr;
+  // Compacted grids: only the LPs still running (rebuilt as LPs stop), and the host copies of
+  // the geometry-0 items the grids' records are made from.
+  CoopItem *act_a = nullptr, *act_t = nullptr;
+  int32_t* act_lp = nullptr;
+  std::vector<int4> items0_a, items0_t;
   int32_t n_act_a = 0, n_act_t = 0, n_act_lp = 0, captured_running = 0;
   bool any_t_long = false;
   double *part_p = nullptr, *part_d = nullptr, *part_er = nullptr, *part_ec = nullptr,
@@ -1283,15 +1316,31 @@ int compact(pdhg_batch* h, bool all) {
   // Items of one product: each XCD's queue holds its LPs' items LP after LP; position p takes
   // the next item of queue p % kXcds, or of the fullest queue once that one is empty.
   auto place = [&](bool rows_of_a) {
-    std::vector<std::vector<int32_t>> q(kXcds);
+    std::vector<std::vector<CoopItem>> q(kXcds);
+    const std::vector<int4>& items0 = rows_of_a ? h->items0_a : h->items0_t;
     size_t total = 0;
     for (int x = 0; x < kXcds; ++x)
       for (const int32_t k : lps_of[x]) {
-        const LpPlan& pd = rows_of_a ? h->desc_host[k].a[0] : h->desc_host[k].t[0];
-        for (int32_t i = 0; i < pd.nitems; ++i) q[x].push_back(pd.item0 + i);
+        const LpDesc& d = h->desc_host[k];
+        const LpPlan& pd = rows_of_a ? d.a[0] : d.t[0];
+        for (int32_t i = 0; i < pd.nitems; ++i) {
+          CoopItem r{};
+          r.item = items0[static_cast<size_t>(pd.item0 + i)];
+          r.b = pd.item0 + i;
+          r.lp = k;
+          r.rp0 = rows_of_a ? d.arp0 : d.trp0;
+          r.lr0 = pd.lr0;
+          r.nnz0 = d.nnz0;
+          r.seg0 = pd.seg0;
+          r.row0 = d.row0;
+          r.col0 = d.col0;
+          r.m = d.m;
+          r.n = d.n;
+          q[x].push_back(r);
+        }
         total += static_cast<size_t>(pd.nitems);
       }
-    std::vector<int32_t> out;
+    std::vector<CoopItem> out;
     out.reserve(total);
     std::array<size_t, kXcds> next{};
     for (size_t p = 0; p < total; ++p) {
@@ -1304,16 +1353,16 @@ int compact(pdhg_batch* h, bool all) {
     }
     return out;
   };
-  const std::vector<int32_t> ia = place(true), it = place(false);
+  const std::vector<CoopItem> ia = place(true), it = place(false);
   h->n_act_lp = static_cast<int32_t>(lps.size());
   h->n_act_a = static_cast<int32_t>(ia.size());
   h->n_act_t = static_cast<int32_t>(it.size());
   if (!lps.empty())
     BHIP_TRY(h, hipMemcpyAsync(h->act_lp, lps.data(), sizeof(int32_t) * lps.size(), hipMemcpyHostToDevice, h->stream));
   if (!ia.empty())
-    BHIP_TRY(h, hipMemcpyAsync(h->act_a, ia.data(), sizeof(int32_t) * ia.size(), hipMemcpyHostToDevice, h->stream));
+    BHIP_TRY(h, hipMemcpyAsync(h->act_a, ia.data(), sizeof(CoopItem) * ia.size(), hipMemcpyHostToDevice, h->stream));
   if (!it.empty())
-    BHIP_TRY(h, hipMemcpyAsync(h->act_t, it.data(), sizeof(int32_t) * it.size(), hipMemcpyHostToDevice, h->stream));
+    BHIP_TRY(h, hipMemcpyAsync(h->act_t, it.data(), sizeof(CoopItem) * it.size(), hipMemcpyHostToDevice, h->stream));
   BHIP_TRY(h, hipStreamSynchronize(h->stream));
   return PDHG_OK;
 }
@@ -1583,6 +1632,8 @@ int pdhg_batch_upload(pdhg_batch* h, int32_t nlp, const int32_t* m, const int32_
     // LP of each geometry-0 item and of each coordinate.
     const PlanSet& pa0 = L.plans[0][0];
     const PlanSet& pt0 = L.plans[1][0];
+    h->items0_a = pa0.items;
+    h->items0_t = pt0.items;
     std::vector<int32_t> ila(pa0.items.size()), ilt(pt0.items.size()), cl_lp(un), rl_lp(um);
     for (int32_t k = 0; k < nlp; ++k) {
       const LpDesc& d = L.desc[k];
