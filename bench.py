@@ -37,7 +37,10 @@ Other fields of the line:
   - `adjoint`: 64 blend-style LPs (m = 5,000, n = 12,000) solved toward relative KKT 2e-10 within
     1,500,000 iterations, then the adjoint: forward and backward seconds, and the envelope error
     of L = c.x* (dL/db against y*) over the LPs with a square active system;
-  - `roofline`: the dominant kernel's algorithmic bytes over its HIP-event-timed duration.
+  - `roofline`: the dominant kernel's algorithmic bytes over its HIP-event-timed duration;
+  - `cpu_threads` (and `c2.cpu_threads`): the oracle on the same LP with one thread and with
+    every host core, setup timed alone, then milliseconds per iteration over a fixed budget
+    (200 iterations here, 50 on the 1M x 2M LP), beside `cpu_baseline`'s extrapolated rate.
 
 Prints one JSON line. One GPU only: --gpus above 1 exits with a message.
 """
@@ -147,6 +150,28 @@ def cpu_baseline(arrays, gpu_iterations, tolerance, budget_s=20.0):
                       f"timed alone, then {iters} iterations at {per_iter*1e3:.1f} ms/iteration, "
                       f"extrapolated to the GPU's {gpu_iterations} iterations "
                       f"({est:.1f}s per LP); status of the sample run {r0['status']}"}
+
+
+def cpu_threads(arrays, tolerance, iterations, all_cores):
+    """The oracle with one thread and with every host core (only its Âᵀy is parallel, as in the
+    reference's threads option) on a fixed iteration budget: setup timed alone, then seconds per
+    iteration, for the CPU rows beside the GPU's."""
+    import oracle as orc
+
+    P = orc.Problem(arrays)
+    out = {"iterations": iterations}
+    for label, threads in (("one_core", 1), ("all_cores", max(1, all_cores))):
+        orc.set_threads(threads)
+        t = time.perf_counter()
+        orc.solve(P, tolerance=tolerance, iteration_limit=0)
+        setup = time.perf_counter() - t
+        t = time.perf_counter()
+        r = orc.solve(P, tolerance=tolerance, iteration_limit=iterations)
+        per = max((time.perf_counter() - t - setup) / iterations, 1e-12)
+        out[label] = {"threads": threads, "setup_seconds": setup, "ms_per_iteration": per * 1e3,
+                      "status": r["status"]}
+    orc.set_threads(1)
+    return out
 
 
 def _single_sample(sh, models, opts, tol):
@@ -278,7 +303,7 @@ def reference_rule_check(sh, seeds=(1, 2), cpu=True):
     return out
 
 
-def c2_check(sh, iteration_cap=30_000, reps=20):
+def c2_check(sh, iteration_cap=30_000, reps=20, cpu_iterations=50):
     """The 1,000,000 x 2,000,000 Zipf LP (nnz 20M, seed 3): mixed precision, FP32 phase to
     relative KKT 1e-6, then the FP64 polish toward 1e-8 under an iteration cap; and a plain FP64
     solve to 1e-6 whose objective the mixed result is compared with."""
@@ -312,6 +337,11 @@ def c2_check(sh, iteration_cap=30_000, reps=20):
     log(f"[bench] c2: fp64 1e-6 it={a.iterations} {ta:.2f}s; mixed 1e-6 it={b.iterations} {tb:.2f}s "
         f"(rel {rel(b):.1e}); mixed+polish it={c.iterations} (fp32 {c.stats['fp32_iterations']}) "
         f"kkt {c.relative_kkt:.2e} {tc:.2f}s")
+    cpu = None
+    if cpu_iterations > 0:
+        cpu = cpu_threads(model.arrays(), 1e-6, cpu_iterations, host_info()[1])
+        log(f"[bench] c2 oracle: {cpu['one_core']['ms_per_iteration']:.1f} ms/iteration on one core, "
+            f"{cpu['all_cores']['ms_per_iteration']:.1f} on {cpu['all_cores']['threads']}")
     return {"workload": "zipf-1m-2m: m = 1,000,000, n = 2,000,000, nnz 20M, seed 3",
             "fp64_1e6": {"seconds": ta, "iterations": a.iterations, "relative_kkt": a.relative_kkt,
                          "status": a.status},
@@ -319,6 +349,7 @@ def c2_check(sh, iteration_cap=30_000, reps=20):
                           "objective_rel_error_vs_fp64": rel(b), "status": b.status,
                           "speedup_vs_fp64": ta / tb if tb > 0 else None},
             "kernels": kernels,
+            "cpu_threads": cpu,
             "mixed_polished_toward_1e8": {
                 "seconds": tc, "iterations": c.iterations, "iteration_cap": iteration_cap,
                 "fp32_iterations": c.stats["fp32_iterations"], "relative_kkt": c.relative_kkt,
@@ -549,7 +580,7 @@ def main():
 
     c2 = None
     if not args.no_c2:
-        c2 = c2_check(sh, reps=args.timing_reps)
+        c2 = c2_check(sh, reps=args.timing_reps, cpu_iterations=0 if args.no_cpu_baseline else 50)
 
     batched = None
     if args.batched_lps > 0:
@@ -560,11 +591,15 @@ def main():
         adjoint = adjoint_check(sh, args.adjoint_lps)
 
     cpu_model, ncpu = host_info()
-    cpu = None
+    cpu = cpu_rows = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline (oracle, one core) on a bounded sample")
         cpu = cpu_baseline(model.arrays(), last.iterations, tol)
         cpu["sample"] += f"; host {cpu_model}, nproc {ncpu}"
+        cpu_rows = cpu_threads(model.arrays(), tol, 200, ncpu)
+        log(f"[bench] oracle on {args.workload}: {cpu_rows['one_core']['ms_per_iteration']:.1f} "
+            f"ms/iteration on one core, {cpu_rows['all_cores']['ms_per_iteration']:.1f} on "
+            f"{cpu_rows['all_cores']['threads']}")
 
     line = {
         "metric": "LPs solved/sec to 1e-6 rel KKT + SpMV HBM GB/s; mixed/batched vs plain FP64",
@@ -598,6 +633,7 @@ def main():
         "resident": resident,
         "reference_rule": reference,
         "cpu_baseline": cpu,
+        "cpu_threads": cpu_rows,
         "host": {"cpu": cpu_model, "nproc": ncpu},
         "mixed": mixed,
         "c2": c2,
