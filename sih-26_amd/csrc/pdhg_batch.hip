@@ -101,11 +101,12 @@ __device__ inline PlanView<V> item_view(const Mat<V>& M, const PlanArrays& P, co
   v.rowptr = M.rowptr + r.rp0;
   v.col = M.col + r.nnz0;
   v.val = M.val + r.nnz0;
-  v.items = P.items;  // an item kernel is handed its item and walks no other
+  // An item kernel is handed its item and walks no other: no item list or tail groups.
+  v.items = P.items;
   v.lr_info = P.lr_info + r.lr0;
   v.lr_groups = P.groups;
   v.seg_part = P.seg_part + 2 * static_cast<int64_t>(r.seg0);
-  v.nitems = 0;  // the item kernels walk only their own item
+  v.nitems = 0;
   v.nlr = 0;
   v.ngroups = 0;
   v.has_group0 = 0;
@@ -414,8 +415,6 @@ __global__ void k_batch_widen(const float* xs, const double* xr, double* xd, int
 template <typename V, typename X>
 struct CoopArgs {
   Args<V, X> a;
-  const int32_t* item_lp_a;  // LP of each geometry-0 work item of Ahat
-  const int32_t* item_lp_t;  // ... of Ahat^T
   const int32_t* col_lp;     // LP of each concatenated column
   const int32_t* row_lp;     // LP of each concatenated row
   const int32_t* coop;       // per LP: 1 when it runs in the cooperative schedule
@@ -991,7 +990,7 @@ struct pdhg_batch {
   Ctl* ctl_host = nullptr;
   long long* clocks = nullptr;
   // Cooperative schedule: LP of every geometry-0 item and coordinate, per-LP mode, partials.
-  int32_t *item_lp_a = nullptr, *item_lp_t = nullptr, *col_lp = nullptr, *row_lp = nullptr;
+  int32_t *col_lp = nullptr, *row_lp = nullptr;
   int32_t* coop_dev = nullptr;
   std::vector<int32_t> coop;  // per LP: 1 cooperative, 0 persistent (geometry in geo)
   int32_t items_a = 0, items_t = 0;
@@ -1157,8 +1156,6 @@ template <typename V, typename X>
 CoopArgs<V, X> coop_args(const pdhg_batch* h) {
   CoopArgs<V, X> c;
   c.a = args_of<V, X>(h, 0);
-  c.item_lp_a = h->item_lp_a;
-  c.item_lp_t = h->item_lp_t;
   c.col_lp = h->col_lp;
   c.row_lp = h->row_lp;
   c.coop = h->coop_dev;
@@ -1629,30 +1626,28 @@ int pdhg_batch_upload(pdhg_batch* h, int32_t nlp, const int32_t* m, const int32_
       (rc = balloc(h, &h->clocks, static_cast<size_t>(nlp))))
     return rc;
   {
-    // LP of each geometry-0 item and of each coordinate.
+    // The geometry-0 items (the cooperative grids' records are made from them) and the LP of
+    // each coordinate.
     const PlanSet& pa0 = L.plans[0][0];
     const PlanSet& pt0 = L.plans[1][0];
     h->items0_a = pa0.items;
     h->items0_t = pt0.items;
-    std::vector<int32_t> ila(pa0.items.size()), ilt(pt0.items.size()), cl_lp(un), rl_lp(um);
+    const size_t ia = pa0.items.size(), it = pt0.items.size();
+    std::vector<int32_t> cl_lp(un), rl_lp(um);
     for (int32_t k = 0; k < nlp; ++k) {
       const LpDesc& d = L.desc[k];
-      for (int32_t q = 0; q < d.a[0].nitems; ++q) ila[d.a[0].item0 + q] = k;
-      for (int32_t q = 0; q < d.t[0].nitems; ++q) ilt[d.t[0].item0 + q] = k;
       for (int32_t j = 0; j < d.n; ++j) cl_lp[d.col0 + j] = k;
       for (int32_t i = 0; i < d.m; ++i) rl_lp[d.row0 + i] = k;
       if (d.t[0].nlr > 0) h->any_t_long = true;
     }
-    h->items_a = static_cast<int32_t>(ila.size());
-    h->items_t = static_cast<int32_t>(ilt.size());
-    if ((rc = bupload(h, &h->item_lp_a, ila.data(), ila.size())) ||
-        (rc = bupload(h, &h->item_lp_t, ilt.data(), ilt.size())) ||
-        (rc = bupload(h, &h->col_lp, cl_lp.data(), cl_lp.size())) ||
+    h->items_a = static_cast<int32_t>(ia);
+    h->items_t = static_cast<int32_t>(it);
+    if ((rc = bupload(h, &h->col_lp, cl_lp.data(), cl_lp.size())) ||
         (rc = bupload(h, &h->row_lp, rl_lp.data(), rl_lp.size())) ||
-        (rc = balloc(h, &h->part_p, ilt.size())) || (rc = balloc(h, &h->part_d, 2 * ila.size())) ||
-        (rc = balloc(h, &h->part_er, 9 * ila.size())) || (rc = balloc(h, &h->part_ec, 9 * ilt.size())) ||
+        (rc = balloc(h, &h->part_p, it)) || (rc = balloc(h, &h->part_d, 2 * ia)) ||
+        (rc = balloc(h, &h->part_er, 9 * ia)) || (rc = balloc(h, &h->part_ec, 9 * it)) ||
         (rc = balloc(h, &h->lr_p, static_cast<size_t>(nlp))) ||
-        (rc = balloc(h, &h->act_a, ila.size())) || (rc = balloc(h, &h->act_t, ilt.size())) ||
+        (rc = balloc(h, &h->act_a, ia)) || (rc = balloc(h, &h->act_t, it)) ||
         (rc = balloc(h, &h->act_lp, static_cast<size_t>(nlp))))
       return rc;
     h->coop.assign(static_cast<size_t>(nlp), 1);

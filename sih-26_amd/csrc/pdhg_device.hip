@@ -698,7 +698,7 @@ struct pdhg_dev {
   int64_t bytes = 0;
   // Device memory comes in blocks of at least kBlockBytes, carved into 256-byte aligned pieces
   // and freed together with the handle: a few hipMalloc and hipFree calls per solve instead of
-  // one per array (about 45), which took the 200k x 500k LP's release from 6-7 ms.
+  // one per array (about 45). The 200k x 500k LP's release went from 6-7 ms to 2 ms.
   std::vector<void*> allocs;
   char* block_next = nullptr;
   size_t block_left = 0;
