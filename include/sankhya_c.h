@@ -61,8 +61,8 @@ double sankhya_infinity(void);
 
 /* sankhya.h:96-188: an empty model, then columns, rows and coefficients. */
 sankhya_model* sankhya_model_create(void);
-/* sankhya.h:99-106: read an MPS file (free or fixed) into the handle, replacing it. A `.lp`
- * file, a quadratic section or gzip input fails with SANKHYA_ERROR_IO; on any failure the
+/* sankhya.h:99-106: read an MPS file (free or fixed, plain or gzip-compressed) into the
+ * handle, replacing it. A `.lp` file or a quadratic section fails with SANKHYA_ERROR_IO; on any failure the
  * handle keeps what it held and sankhya_last_error() names the file and line. */
 sankhya_status sankhya_model_read(sankhya_model* model, const char* path);
 sankhya_status sankhya_model_set_maximize(sankhya_model* model, int maximize);

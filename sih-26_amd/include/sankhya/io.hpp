@@ -4,8 +4,8 @@
 // Reading (reference include/sankhya/io.hpp:19-53): MPS in its free and fixed dialects. A read
 // returns a status, never throws: a malformed file is ordinary input, and every message names
 // the file and the 1-based line. A reader validates the model before it reports success.
-// CPLEX LP files, quadratic (QPS) sections and gzip input are refused with a message naming
-// the reason: the engine here solves linear programs, and the build links no zlib.
+// CPLEX LP files and quadratic (QPS) sections are refused with a message naming the reason:
+// the engine here solves linear programs. Gzip-compressed files are read through zlib.
 //
 // The solution file, the engine's certificate output (reference: include/sankhya/io.hpp:56-72,
 // format of src/io/writer.cpp:121-284):

@@ -14,12 +14,14 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
-#include <fstream>
+#include <cstring>
 #include <string>
 #include <unordered_map>
 #include <unordered_set>
 #include <utility>
 #include <vector>
+
+#include <zlib.h>
 
 #include "sankhya/io.hpp"
 #include "text_format.hpp"
@@ -88,17 +90,55 @@ bool parse_number(std::string text, double* out) {
   return true;
 }
 
+// The lines of a plain or gzip-compressed file (the reference reads `.gz` through zlib,
+// src/io/line_reader.cpp:43-60); zlib passes an uncompressed file through as it is.
+class LineSource {
+ public:
+  explicit LineSource(const std::string& path) : gz_(gzopen(path.c_str(), "rb")) {
+    if (gz_ != nullptr) gzbuffer(gz_, 1 << 17);
+  }
+  ~LineSource() {
+    if (gz_ != nullptr) gzclose(gz_);
+  }
+  LineSource(const LineSource&) = delete;
+  LineSource& operator=(const LineSource&) = delete;
+  bool open() const { return gz_ != nullptr; }
+  // The next line without its newline; false at the end of the data or on a read error.
+  bool next(std::string* line) {
+    line->clear();
+    char buf[4096];
+    while (gzgets(gz_, buf, sizeof buf) != nullptr) {
+      const std::size_t len = std::strlen(buf);
+      line->append(buf, len);
+      if (len > 0 && buf[len - 1] == '\n') {
+        line->pop_back();
+        return true;
+      }
+    }
+    return !line->empty();
+  }
+  // Corrupt or truncated compressed data (a clean end of the data reads as no error).
+  bool failed() const {
+    int code = Z_OK;
+    gzerror(gz_, &code);
+    return code != Z_OK && code != Z_STREAM_END;
+  }
+
+ private:
+  gzFile gz_;
+};
+
 class MpsReader {
  public:
   MpsReader(std::string path, MpsFormat dialect)
       : path_(std::move(path)), fixed_(dialect == MpsFormat::kFixed) {}
 
   ReadResult read(Model* model) {
-    std::ifstream in(path_);
-    if (!in) return ReadResult::failure(format("{}: cannot open the file", path_));
+    LineSource in(path_);
+    if (!in.open()) return ReadResult::failure(format("{}: cannot open the file", path_));
     std::string line, err;
     bool ended = false;
-    while (std::getline(in, line)) {
+    while (in.next(&line)) {
       ++line_no_;
       if (!line.empty() && line.back() == '\r') line.pop_back();
       if (line.empty() || line[0] == '*') continue;
@@ -126,6 +166,9 @@ class MpsReader {
       }
       if (!data(tok, &err)) return fail(err);
     }
+    if (!ended && in.failed())
+      return ReadResult::failure(format("{}: corrupt or truncated compressed data after line {}",
+                                        path_, line_no_));
     if (!ended) return ReadResult::failure(format("{}: the file ends without an ENDATA record", path_));
     return finish(model);
   }
@@ -521,8 +564,6 @@ bool parse_mps_format(const std::string& text, MpsFormat* out) noexcept {
 ReadResult read_mps(const std::string& path, Model* model, MpsFormat format_wanted,
                     MpsFormat* format_used) {
   if (model == nullptr) return ReadResult::failure(path + ": no model to read into");
-  if (ends_with(path, ".gz"))
-    return ReadResult::failure(path + ": gzip input needs zlib, which this build does not link");
   if (format_wanted != MpsFormat::kAuto) {
     if (format_used != nullptr) *format_used = format_wanted;
     return MpsReader(path, format_wanted).read(model);
